@@ -1,0 +1,321 @@
+// C-ABI: workspace lifetime, eager warm-up, graph capture and replay.
+// Declarations and reference citations: include/peel355.h.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/peel355.h"
+#include "workspace.hpp"
+
+namespace peel {
+int subdivide_blocks_per_cu();
+}
+
+using namespace peel;
+
+struct PeelWorkspace {
+  Ws w;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool warmed = false;
+  int device = 0;
+  std::vector<void *> allocations;
+};
+
+static thread_local std::string g_error;
+
+static int fail(int code, const std::string &message) {
+  g_error = message;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(-(int)_e, std::string(#expr) + ": " + hipGetErrorString(_e));        \
+  } while (0)
+
+template <class T>
+static int alloc(PeelWorkspace *ws, T *&ptr, long count) {
+  if (count < 1) count = 1;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, sizeof(T) * (size_t)count);
+  if (e != hipSuccess)
+    return fail(-(int)e, "hipMalloc of " + std::to_string(sizeof(T) * (size_t)count) +
+                             " bytes: " + hipGetErrorString(e));
+  e = hipMemset(p, 0, sizeof(T) * (size_t)count);
+  if (e != hipSuccess) return fail(-(int)e, std::string("hipMemset: ") + hipGetErrorString(e));
+  ws->allocations.push_back(p);
+  ptr = (T *)p;
+  return 0;
+}
+
+#define ALLOC(ptr, n)                          \
+  do {                                         \
+    int _r = alloc(ws, ptr, (long)(n));        \
+    if (_r) {                                  \
+      peel_destroy(ws);                        \
+      return _r;                               \
+    }                                          \
+  } while (0)
+
+static int alloc_hierarchy(PeelWorkspace *ws, Hierarchy &h, int leaves) {
+  h.size = pow2_at_least(leaves < 1 ? 1 : leaves);
+  int r;
+  if ((r = alloc(ws, h.keys, h.size))) return r;
+  if ((r = alloc(ws, h.alt_keys, h.size))) return r;
+  if ((r = alloc(ws, h.ids, h.size))) return r;
+  if ((r = alloc(ws, h.alt_ids, h.size))) return r;
+  if ((r = alloc(ws, h.nodes, 2L * h.size))) return r;
+  return 0;
+}
+
+extern "C" {
+
+const char *peel_last_error(void) { return g_error.c_str(); }
+
+const char *peel_version(void) { return "peel355 0.1 gfx950"; }
+
+int peel_destroy(PeelWorkspace *ws) {
+  if (!ws) return 0;
+  if (ws->exec) hipGraphExecDestroy(ws->exec);
+  if (ws->graph) hipGraphDestroy(ws->graph);
+  if (ws->stream) hipStreamSynchronize(ws->stream);
+  for (void *p : ws->allocations) hipFree(p);
+  if (ws->stream) hipStreamDestroy(ws->stream);
+  delete ws;
+  return 0;
+}
+
+int peel_create(const PeelDesc *d, PeelWorkspace **out) {
+  if (!d || !out) return fail(1, "null argument");
+  *out = nullptr;
+  if (d->num_vertices < 0 || d->num_faces < 0 || d->num_vertices > (1L << 30) ||
+      d->num_faces > (1L << 27))
+    return fail(2, "vertex or face count out of range");
+  if ((d->num_vertices && !d->vertices) || (d->num_faces && !d->faces) || !d->camera || !d->settings)
+    return fail(3, "null device array");
+  if (d->frontier < 1 || d->regions < 1 || d->hits < 1 || d->vertices_cap < 1 || d->triangles < 1 ||
+      d->rounds < 1 || d->layer_capacity < 1 || d->edge_vertices < 1)
+    return fail(4, "Capacities must be positive");
+  if (d->edge_vertices > 128) return fail(5, "edge_vertices must be <= 128");
+  if (d->layers < 0 || d->layers > d->layer_capacity)
+    return fail(6, "layers must be zero (all), or fit in the layer capacity");
+  if (d->route != PEEL_ROUTE_EVENTS && d->route != PEEL_ROUTE_LAYERED) return fail(7, "unknown route");
+  PeelWorkspace *ws = new PeelWorkspace();
+  HIP_TRY(hipGetDevice(&ws->device));
+  Ws &w = ws->w;
+  std::memset(&w, 0, sizeof(Ws));
+  w.vertices = d->vertices;
+  w.faces = d->faces;
+  w.camera = d->camera;
+  w.settings = d->settings;
+  w.nv = (int)d->num_vertices;
+  w.nf = (int)d->num_faces;
+  w.cap = Caps{d->frontier, d->regions, d->hits, d->vertices_cap, d->triangles, d->rounds,
+               d->layer_capacity, d->edge_vertices};
+  w.limit = d->layers ? d->layers : d->layer_capacity;
+  w.all_layers = d->layers == 0;
+  w.share_regions = (d->flags & PEEL_SHARE_REGIONS) != 0;
+  w.propagate_ranks = (d->flags & PEEL_PROPAGATE_RANKS) != 0;
+  int nf1 = w.nf > 0 ? w.nf : 1, nv1 = w.nv > 0 ? w.nv : 1;
+  ALLOC(w.status, 16);
+  ALLOC(w.metrics, M_COUNT);
+  ALLOC(w.control, C_COUNT);
+  ALLOC(w.inverse, 16);
+  ALLOC(w.canonical, nv1);
+  w.vhash_size = pow2_at_least(2 * nv1);
+  ALLOC(w.vhash, w.vhash_size);
+  w.ehash_size = pow2_at_least(6 * nf1);
+  ALLOC(w.ekeys, w.ehash_size);
+  ALLOC(w.ecount, w.ehash_size);
+  ALLOC(w.efirst, w.ehash_size);
+  ALLOC(w.esecond, w.ehash_size);
+  ALLOC(w.masks, 3L * nf1);
+  ALLOC(w.src_xy, (long)SRCV * nf1);
+  ALLOC(w.src_n, nf1);
+  ALLOC(w.scores, nf1);
+  ALLOC(w.facing, nf1);
+  ALLOC(w.clipped, nf1);
+  ALLOC(w.src_boxes, nf1);
+  w.nseg = 8 * nf1;
+  ALLOC(w.seg_a, w.nseg);
+  ALLOC(w.seg_b, w.nseg);
+  ALLOC(w.seg_near, 2L * w.nseg);
+  ALLOC(w.seg_rep, w.nseg);
+  w.shash_size = pow2_at_least(2 * w.nseg);
+  ALLOC(w.shash, w.shash_size);
+  ALLOC(w.seg_boxes, w.nseg);
+  int r;
+  if ((r = alloc_hierarchy(ws, w.source_tree, nf1)) || (r = alloc_hierarchy(ws, w.contour_tree, w.nseg)) ||
+      (r = alloc_hierarchy(ws, w.point_tree, w.cap.vertices))) {
+    peel_destroy(ws);
+    return r;
+  }
+  long F = w.cap.frontier, R = w.cap.regions;
+  ALLOC(w.fr_xy, F * MAXV);
+  ALLOC(w.fr_n, F);
+  ALLOC(w.fr_src, F);
+  ALLOC(w.fr_node, F);
+  ALLOC(w.fr_gen, F);
+  ALLOC(w.fr_ready, F);
+  ALLOC(w.out_xy, R * MAXV);
+  ALLOC(w.out_n, R);
+  ALLOC(w.out_src, R);
+  ALLOC(w.out_layer, R);
+  ALLOC(w.samples, R);
+  ALLOC(w.reps, R);
+  w.qhash_size = pow2_at_least(2 * w.cap.regions);
+  ALLOC(w.qhash, w.qhash_size);
+  ALLOC(w.groups, R);
+  ALLOC(w.group_counts, R);
+  ALLOC(w.group_offsets, R);
+  ALLOC(w.heavy, R);
+  long H = w.cap.hits;
+  ALLOC(w.hit_depth, H);
+  ALLOC(w.hit_src, H);
+  ALLOC(w.hit_group, H);
+  ALLOC(w.hkeys, H);
+  ALLOC(w.hkeys_alt, H);
+  ALLOC(w.hids, H);
+  ALLOC(w.hids_alt, H);
+  long BV = w.cap.vertices, T = w.cap.triangles;
+  ALLOC(w.vcounts, R);
+  ALLOC(w.voffsets, R);
+  ALLOC(w.bverts, BV);
+  ALLOC(w.blayers, BV);
+  ALLOC(w.tasks, BV);
+  ALLOC(w.point_boxes, BV);
+  ALLOC(w.tri_uv, T * 6);
+  ALLOC(w.tri_world, T * 9);
+  ALLOC(w.tri_src, T);
+  ALLOC(w.tri_layer, T);
+  long sort_cap = H;
+  sort_cap = std::max<long>(sort_cap, w.source_tree.size);
+  sort_cap = std::max<long>(sort_cap, w.contour_tree.size);
+  sort_cap = std::max<long>(sort_cap, w.point_tree.size);
+  w.sort.max_blocks = (int)((sort_cap + RS_TILE - 1) / RS_TILE);
+  ALLOC(w.sort.hist, 256L * w.sort.max_blocks);
+  ALLOC(w.sort.flags, RS_MAX_PASSES + 1);
+  w.scan.max_blocks = (int)((R + 4095) / 4096) + 1;
+  ALLOC(w.scan.block_sums, w.scan.max_blocks);
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device));
+  int per_cu = subdivide_blocks_per_cu();
+  w.persistent_blocks = std::max(1, cus * std::max(1, per_cu));
+  hipError_t e = hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    peel_destroy(ws);
+    return fail(-(int)e, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    peel_destroy(ws);
+    return fail(-(int)e, std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
+  }
+  *out = ws;
+  return 0;
+}
+
+int peel_enqueue(PeelWorkspace *ws) {
+  if (!ws) return fail(1, "null workspace");
+  const Ws &w = ws->w;
+  enqueue_prepare(w, ws->stream);
+  enqueue_subdivide(w, ws->stream);
+  enqueue_rank(w, ws->stream);
+  enqueue_mesh(w, ws->stream);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int peel_warmup(PeelWorkspace *ws) {
+  if (!ws) return fail(1, "null workspace");
+  hipStreamCaptureStatus st;
+  HIP_TRY(hipStreamIsCapturing(ws->stream, &st));
+  if (st != hipStreamCaptureStatusNone) return fail(8, "Warm-up must run before graph capture");
+  int r = peel_enqueue(ws);
+  if (r) return r;
+  HIP_TRY(hipStreamSynchronize(ws->stream));
+  ws->warmed = true;
+  return 0;
+}
+
+int peel_capture(PeelWorkspace *ws) {
+  if (!ws) return fail(1, "null workspace");
+  if (!ws->warmed) {
+    int r = peel_warmup(ws);
+    if (r) return r;
+  }
+  if (ws->exec) {
+    hipGraphExecDestroy(ws->exec);
+    ws->exec = nullptr;
+  }
+  if (ws->graph) {
+    hipGraphDestroy(ws->graph);
+    ws->graph = nullptr;
+  }
+  HIP_TRY(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal));
+  int r = peel_enqueue(ws);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(ws->stream, &g);
+  if (r) {
+    if (g) hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return fail(-(int)e, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  ws->graph = g;
+  HIP_TRY(hipGraphInstantiate(&ws->exec, ws->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+int peel_launch(PeelWorkspace *ws) {
+  if (!ws) return fail(1, "null workspace");
+  if (!ws->exec) {
+    int r = peel_capture(ws);
+    if (r) return r;
+  }
+  HIP_TRY(hipGraphLaunch(ws->exec, ws->stream));
+  return 0;
+}
+
+int peel_synchronize(PeelWorkspace *ws) {
+  if (!ws) return fail(1, "null workspace");
+  HIP_TRY(hipStreamSynchronize(ws->stream));
+  return 0;
+}
+
+int peel_read_status(PeelWorkspace *ws, int32_t *status, int64_t *metrics) {
+  if (!ws) return fail(1, "null workspace");
+  HIP_TRY(hipStreamSynchronize(ws->stream));
+  if (status) HIP_TRY(hipMemcpy(status, ws->w.status, 16 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (metrics)
+    HIP_TRY(hipMemcpy(metrics, ws->w.metrics, M_COUNT * sizeof(int64_t), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int peel_get_buffers(PeelWorkspace *ws, PeelBuffers *b) {
+  if (!ws || !b) return fail(1, "null argument");
+  const Ws &w = ws->w;
+  b->output_xy = (double *)w.out_xy;
+  b->output_n = w.out_n;
+  b->output_sources = w.out_src;
+  b->output_layers = w.out_layer;
+  b->regions = w.cap.regions;
+  b->max_vertices = MAXV;
+  b->triangles_uv = w.tri_uv;
+  b->triangles_world = w.tri_world;
+  b->triangle_sources = w.tri_src;
+  b->triangle_layers = w.tri_layer;
+  b->triangles = w.cap.triangles;
+  b->status = w.status;
+  b->metrics = (int64_t *)w.metrics;
+  b->metrics_count = M_COUNT;
+  b->canonical = w.canonical;
+  b->stream = (void *)ws->stream;
+  return 0;
+}
+
+}  // extern "C"

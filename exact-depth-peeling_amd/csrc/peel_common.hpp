@@ -1,0 +1,193 @@
+// Shared device definitions for the gfx950 depth peeler.
+//
+// Layout conventions (HBM):
+//  * Polygon rings are stored vertex-major, ring[j * capacity + slot], so the
+//    64 lanes of a wave that work on 64 consecutive slots read vertex j of all
+//    of them in one coalesced 1 KiB transaction (double2 per lane).
+//  * Hierarchy nodes are 32-byte {float4 lo, float4 hi} records of an implicit
+//    complete binary tree (node 1 is the root, leaves at [size, 2 size)),
+//    matching the stackless traversal of warp_peel/kernels.py:24-64.
+//  * Every pool is preallocated; counts live on the device and every kernel
+//    reads its own trip count from device memory, so a captured graph needs no
+//    host reads or reallocation between replays.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace peel {
+
+constexpr int MAXV = 32;      // ring capacity (warp_peel/kernels.py:10)
+constexpr int SRCV = 8;       // triangle + near plane + 4 window planes
+constexpr double EPS = 1.0e-10;  // warp_peel/kernels.py:11
+constexpr float PAD = 2.0e-7f;   // box padding (warp_peel/kernels.py:235-250)
+
+// status[16] slots and error bits: warp_peel/kernels.py:17-21, events.py:811-814.
+enum Status : int {
+  ST_CUR = 0,
+  ST_NEXT = 1,
+  ST_POLYS = 2,
+  ST_TRIS = 3,
+  ST_LAYERS = 4,
+  ST_PASS = 5,
+  ST_BVERTS = 6,
+  ST_TIES = 8,
+  ST_ERR = 9,
+  ST_MAXH = 10,
+  ST_PEAK = 12,
+};
+enum Err : int {
+  ERR_POLY = 1,
+  ERR_PIECES = 2,
+  ERR_RESIDUAL = 4,
+  ERR_OUTPUT = 8,
+  ERR_VERTS = 16,
+  ERR_TRIS = 32,
+  ERR_LAYERS = 64,
+  ERR_EDGE = 128,
+  ERR_INPUT = 256,
+  ERR_HITS = 1024,
+  ERR_ROUNDS = 2048,
+};
+// metrics[]: the 19 counters of warp_peel/events.py:819-843, then this build's own.
+enum Metric : int {
+  M_VISITS = 0,
+  M_CONTOUR_TESTS = 1,
+  M_SPLITS = 2,
+  M_RAY_GROUPS = 3,
+  M_RAY_TESTS = 4,
+  M_RAY_HITS = 5,
+  M_ROUNDS = 6,
+  M_UNIQUE = 7,
+  M_HEAVY = 8,
+  M_BEAM_ATTEMPTS = 9,
+  M_BEAMS = 10,
+  M_BEAM_TESTS = 11,
+  M_HEAVY_BEAMS = 12,
+  M_COMPONENTS = 13,
+  M_CONNECTIONS = 14,
+  M_CONNECTION_TESTS = 15,
+  M_MERGED = 16,
+  M_MATCHED = 17,
+  M_HASH_MISSES = 18,
+  M_FILTER_TESTS = 19,   // predicates decided by the binary32 filter
+  M_FALLBACKS = 20,      // predicates that needed binary64
+  M_SLOTS = 21,          // frontier slots used (seeds + 2 per split)
+  M_COUNT = 24,
+};
+
+// Control words (device) used by persistent kernels and on-device sort passes.
+enum Control : int {
+  C_ALLOC = 0,    // next free frontier slot
+  C_HEAD = 1,     // next ticket
+  C_PENDING = 2,  // live frontier tasks
+  C_HEAVY = 3,
+  C_COUNT = 64,
+};
+
+struct __align__(16) Box {
+  float4 lo, hi;
+};
+
+struct Tree {
+  const Box *nodes;  // [2 * leaves]
+  const int *ids;    // [leaves] sorted leaf ids
+  int leaves;        // power of two
+};
+
+// ---------------------------------------------------------------------------
+// small helpers
+__device__ __forceinline__ double cross3(double2 a, double2 b, double2 c) {
+  double ux = b.x - a.x, uy = b.y - a.y, vx = c.x - a.x, vy = c.y - a.y;
+  return ux * vy - uy * vx;
+}
+__device__ __forceinline__ double value(double4 s, double2 x) { return s.x * x.x + s.y * x.y + s.z; }
+__device__ __forceinline__ bool lexless(double2 a, double2 b) {
+  return a.x < b.x || (a.x == b.x && a.y < b.y);
+}
+__device__ __forceinline__ bool same(double2 a, double2 b) { return a.x == b.x && a.y == b.y; }
+__device__ __forceinline__ double dlength(double2 d) { return sqrt(d.x * d.x + d.y * d.y); }
+
+__device__ __forceinline__ void set_error(int *status, int bit) { atomicOr(status + ST_ERR, bit); }
+__device__ __forceinline__ int load_i(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int load_acquire(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_release(int *p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ long long load_ll(const long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Order-preserving map of binary64 to unsigned 64-bit keys; -0.0 and +0.0 map
+// to the same key (sorting must agree with ==, warp_peel tests/gpu
+// test_peeler.py:305-319).
+__device__ __forceinline__ uint64_t dkey(double x) {
+  if (x == 0.0) x = 0.0;
+  uint64_t b = (uint64_t)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double unkey(uint64_t k) {
+  uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
+// Atomic min on binary64 through the order-preserving key (values are finite).
+__device__ __forceinline__ void atomic_min_key(unsigned long long *p, double x) {
+  atomicMin(p, (unsigned long long)dkey(x));
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  // splitmix64 finaliser
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ uint64_t dbits(double x) {
+  if (x == 0.0) x = 0.0;
+  return (uint64_t)__double_as_longlong(x);
+}
+
+// ---------------------------------------------------------------------------
+// Stackless traversal of the implicit complete tree (warp_peel/kernels.py:32-64,
+// event_kernels.py:172-193): returns the next leaf id overlapping [lo, hi] in
+// depth-first order from `node`, and the continuation node (0 when exhausted).
+__device__ __forceinline__ int successor(int node) {
+  while (node > 0 && (node & 1) != 0) node >>= 1;
+  if (node > 0) node += 1;
+  return node;
+}
+__device__ __forceinline__ bool overlaps(const Box &b, float4 lo, float4 hi) {
+  return b.lo.x <= hi.x && b.lo.y <= hi.y && b.lo.z <= hi.z && b.hi.x >= lo.x && b.hi.y >= lo.y &&
+         b.hi.z >= lo.z;
+}
+__device__ __forceinline__ int query_next(const Tree &t, float4 lo, float4 hi, int &node, int end = 0) {
+  while (node > 0 && node != end) {
+    Box b = t.nodes[node];
+    if (overlaps(b, lo, hi)) {
+      if (node < t.leaves) {
+        node = 2 * node;
+      } else {
+        int found = t.ids[node - t.leaves];
+        node = successor(node);
+        return found;
+      }
+    } else {
+      node = successor(node);
+    }
+  }
+  return -1;
+}
+
+// Padded binary32 box of a ring evaluated with its depth plane
+// (warp_peel/kernels.py:235-250). Conversions round to nearest.
+__device__ __forceinline__ void pad_box(double lx, double ly, double lz, double hx, double hy, double hz,
+                                        float4 &lo, float4 &hi) {
+  double pz = 2.0e-7 * fmax(1.0, fmax(fabs(lz), fabs(hz)));
+  lo = make_float4((float)(lx - 2.0e-7), (float)(ly - 2.0e-7), (float)(lz - pz), 0.f);
+  hi = make_float4((float)(hx + 2.0e-7), (float)(hy + 2.0e-7), (float)(hz + pz), 0.f);
+}
+
+}  // namespace peel

@@ -1,0 +1,515 @@
+// Stage 1: reset, input validation, exact welding, projection, contour
+// topology, contour segment grouping and the two static hierarchies.
+//
+// Reference semantics (file:line in alecjacobson/exact-depth-peeling):
+//  reset            warp_peel/kernels.py:253-266
+//  validate_input   warp_peel/kernels.py:309-350
+//  weld             warp_peel/kernels.py:269-306 (stable-sort first equal vertex)
+//  project          warp_peel/kernels.py:353-468
+//  edge topology    warp_peel/kernels.py:471-536
+//  contour segments warp_peel/event_kernels.py:388-501
+//
+// Welding and the two groupings use open-addressing hash tables instead of the
+// reference's chains of stable radix sorts: the representative of a group is
+// the minimum index (atomicMin), which is exactly "first equal element of a
+// stable sort that starts from the identity permutation", in one pass.
+#include "workspace.hpp"
+
+namespace peel {
+
+__global__ void k_reset(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 16) w.status[i] = 0;
+  if (i < M_COUNT) w.metrics[i] = 0;
+  if (i < C_COUNT) w.control[i] = 0;
+  for (int j = i; j < w.vhash_size; j += gridDim.x * blockDim.x) w.vhash[j] = -1;
+  for (int j = i; j < w.ehash_size; j += gridDim.x * blockDim.x) {
+    w.ekeys[j] = ~0ull;
+    w.ecount[j] = 0;
+  }
+  for (int j = i; j < w.shash_size; j += gridDim.x * blockDim.x) w.shash[j] = -1;
+  for (int j = i; j < 2 * w.nseg; j += gridDim.x * blockDim.x) w.seg_near[j] = ~0ull;
+}
+
+__global__ void k_validate(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (i < w.nv) {
+    for (int j = 0; j < 3; ++j) bad |= !isfinite(w.vertices[3 * i + j]);
+  }
+  if (i < w.nf) {
+    for (int j = 0; j < 3; ++j) {
+      int f = w.faces[3 * i + j];
+      bad |= f < 0 || f >= w.nv;
+    }
+  }
+  if (i == 0) {
+    const double *m = w.camera;
+    for (int k = 0; k < 16; ++k) bad |= !isfinite(m[k]);
+    for (int k = 0; k < 6; ++k) bad |= !isfinite(w.settings[k]);
+    // determinant by cofactor expansion along the last row
+    double det = 0.0;
+    {
+      double a[4][4];
+      for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) a[r][c] = m[4 * r + c];
+      for (int c = 0; c < 4; ++c) {
+        double minor[3][3];
+        for (int r = 0; r < 3; ++r) {
+          int cc = 0;
+          for (int k = 0; k < 4; ++k)
+            if (k != c) minor[r][cc++] = a[r][k];
+        }
+        double d3 = minor[0][0] * (minor[1][1] * minor[2][2] - minor[1][2] * minor[2][1]) -
+                    minor[0][1] * (minor[1][0] * minor[2][2] - minor[1][2] * minor[2][0]) +
+                    minor[0][2] * (minor[1][0] * minor[2][1] - minor[1][1] * minor[2][0]);
+        double sign = ((3 + c) & 1) ? -1.0 : 1.0;
+        det += sign * a[3][c] * d3;
+      }
+    }
+    bad |= det == 0.0 || m[12] != 0.0 || m[13] != 0.0 || m[14] != 0.0 || m[15] != 1.0;
+    const double *s = w.settings;
+    bad |= s[2] <= s[0] || s[3] <= s[1] || s[5] <= 0.0 || (s[4] != 0.0 && s[4] != 1.0);
+    // inverse affine camera for lifting (rows 0..2 of [R t; 0 1]^-1)
+    double R[3][3], inv[3][3];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) R[r][c] = m[4 * r + c];
+    double d = R[0][0] * (R[1][1] * R[2][2] - R[1][2] * R[2][1]) -
+               R[0][1] * (R[1][0] * R[2][2] - R[1][2] * R[2][0]) +
+               R[0][2] * (R[1][0] * R[2][1] - R[1][1] * R[2][0]);
+    inv[0][0] = (R[1][1] * R[2][2] - R[1][2] * R[2][1]) / d;
+    inv[0][1] = (R[0][2] * R[2][1] - R[0][1] * R[2][2]) / d;
+    inv[0][2] = (R[0][1] * R[1][2] - R[0][2] * R[1][1]) / d;
+    inv[1][0] = (R[1][2] * R[2][0] - R[1][0] * R[2][2]) / d;
+    inv[1][1] = (R[0][0] * R[2][2] - R[0][2] * R[2][0]) / d;
+    inv[1][2] = (R[0][2] * R[1][0] - R[0][0] * R[1][2]) / d;
+    inv[2][0] = (R[1][0] * R[2][1] - R[1][1] * R[2][0]) / d;
+    inv[2][1] = (R[0][1] * R[2][0] - R[0][0] * R[2][1]) / d;
+    inv[2][2] = (R[0][0] * R[1][1] - R[0][1] * R[1][0]) / d;
+    for (int r = 0; r < 3; ++r) {
+      double t = -(inv[r][0] * m[3] + inv[r][1] * m[7] + inv[r][2] * m[11]);
+      for (int c = 0; c < 3; ++c) w.inverse[4 * r + c] = inv[r][c];
+      w.inverse[4 * r + 3] = t;
+    }
+    w.inverse[12] = w.inverse[13] = w.inverse[14] = 0.0;
+    w.inverse[15] = 1.0;
+  }
+  if (bad) set_error(w.status, ERR_INPUT);
+}
+
+// ---------------------------------------------------------------------------
+// Exact welding (warp_peel/kernels.py:290-306): canonical[v] = smallest index
+// of a vertex with equal coordinates (+0 == -0).
+__device__ __forceinline__ uint64_t vertex_hash(double x, double y, double z) {
+  return mix64(dbits(x) ^ mix64(dbits(y) + 0x9e3779b97f4a7c15ull) ^
+               mix64(dbits(z) + 0x3c6ef372fe94f82aull));
+}
+
+__device__ __forceinline__ bool same_vertex(const double *v, int a, int b) {
+  return v[3 * a] == v[3 * b] && v[3 * a + 1] == v[3 * b + 1] && v[3 * a + 2] == v[3 * b + 2];
+}
+
+__global__ void k_weld_insert(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nv) return;
+  const double *v = w.vertices;
+  int mask = w.vhash_size - 1;
+  int slot = (int)(vertex_hash(v[3 * i], v[3 * i + 1], v[3 * i + 2]) & (uint64_t)mask);
+  for (int probe = 0; probe < w.vhash_size; ++probe) {
+    int cur = atomicCAS(&w.vhash[slot], -1, i);
+    if (cur < 0) return;
+    if (same_vertex(v, cur, i)) {
+      atomicMin(&w.vhash[slot], i);
+      return;
+    }
+    slot = (slot + 1) & mask;
+  }
+}
+
+__global__ void k_weld_resolve(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nv) return;
+  const double *v = w.vertices;
+  int mask = w.vhash_size - 1;
+  int slot = (int)(vertex_hash(v[3 * i], v[3 * i + 1], v[3 * i + 2]) & (uint64_t)mask);
+  int result = i;
+  for (int probe = 0; probe < w.vhash_size; ++probe) {
+    int cur = w.vhash[slot];
+    if (cur < 0) break;
+    if (same_vertex(v, cur, i)) {
+      result = cur;
+      break;
+    }
+    slot = (slot + 1) & mask;
+  }
+  w.canonical[i] = result;
+}
+
+// ---------------------------------------------------------------------------
+// Ring helpers for projection (rings of at most 8 vertices, kept in registers
+// by full unrolling). Semantics of warp_peel/kernels.py:146-232.
+struct Ring8 {
+  double2 v[SRCV + 1];
+  int n;
+};
+
+__device__ __forceinline__ void ring_append(Ring8 &r, double2 x) {
+  if (r.n < SRCV + 1) r.v[r.n] = x;
+  r.n += 1;
+}
+
+__device__ void ring_clean(Ring8 &p) {
+  // warp_peel/kernels.py:146-176
+  if (p.n > SRCV + 1) return;
+  Ring8 q;
+  q.n = 0;
+  for (int j = 0; j < p.n; ++j) {
+    double2 x = p.v[j];
+    if (q.n == 0 || dlength(make_double2(x.x - q.v[q.n - 1].x, x.y - q.v[q.n - 1].y)) > EPS * 0.01)
+      ring_append(q, x);
+  }
+  if (q.n > 1 && dlength(make_double2(q.v[0].x - q.v[q.n - 1].x, q.v[0].y - q.v[q.n - 1].y)) <= EPS * 0.01)
+    q.n -= 1;
+  if (q.n < 3) {
+    q.n = 0;
+  } else {
+    double2 origin = q.v[0];
+    double area2 = 0.0, perimeter = 0.0;
+    for (int j = 0; j < q.n; ++j) {
+      double2 a = q.v[j], b = q.v[(j + 1) % q.n];
+      area2 += cross3(origin, a, b);
+      perimeter += fabs(b.x - a.x) + fabs(b.y - a.y);
+    }
+    if (area2 <= EPS * perimeter) q.n = 0;
+  }
+  p = q;
+}
+
+// Keep the part with s(x) >= 0 (inside of split(), warp_peel/kernels.py:184-203).
+__device__ void ring_keep(Ring8 &p, double sx, double sy, double sc) {
+  Ring8 in;
+  in.n = 0;
+  for (int j = 0; j < p.n && j < SRCV + 1; ++j) {
+    double2 a = p.v[j], b = p.v[(j + 1) % p.n];
+    double da = sx * (a.x - 0.0) + sy * (a.y - 0.0) + sc;
+    double db = sx * (b.x - 0.0) + sy * (b.y - 0.0) + sc;
+    if (da >= 0.0) ring_append(in, a);
+    if ((da > 0.0 && db < 0.0) || (da < 0.0 && db > 0.0)) {
+      double t = da / (da - db);
+      ring_append(in, make_double2(a.x + t * (b.x - a.x), a.y + t * (b.y - a.y)));
+    }
+  }
+  ring_clean(in);
+  p = in;
+}
+
+__global__ void k_project(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nf) return;
+  int nf = w.nf;
+  Box empty;
+  empty.lo = make_float4(1e30f, 1e30f, 1e30f, 0.f);
+  empty.hi = make_float4(-1e30f, -1e30f, -1e30f, 0.f);
+  if (w.status[ST_ERR] != 0) {
+    w.src_n[i] = 0;
+    w.src_boxes[i] = empty;
+    w.scores[i] = make_double4(0, 0, 0, 0);
+    w.facing[i] = 0;
+    w.clipped[i] = 0;
+    return;
+  }
+  const double *m = w.camera;
+  const double *s = w.settings;
+  bool persp = s[4] != 0.0;
+  double r[SRCV][3];
+  int count = 3;
+  bool bad = false;
+  for (int j = 0; j < 3; ++j) {
+    int vi = w.faces[3 * i + j];
+    double x = w.vertices[3 * vi], y = w.vertices[3 * vi + 1], z = w.vertices[3 * vi + 2];
+    for (int k = 0; k < 3; ++k) {
+      r[j][k] = m[4 * k] * x + m[4 * k + 1] * y + m[4 * k + 2] * z + m[4 * k + 3] * 1.0;
+      bad |= !isfinite(r[j][k]);
+    }
+  }
+  if (bad) set_error(w.status, ERR_INPUT);
+  int was_clipped = 0;
+  if (persp) {
+    // perspective near clip (warp_peel/kernels.py:392-415)
+    double near_ring[SRCV][3];
+    count = 0;
+    for (int j = 0; j < 3; ++j) {
+      const double *a = r[j], *b = r[(j + 1) % 3];
+      double da = a[2] - s[5], db = b[2] - s[5];
+      if (da >= 0.0) {
+        for (int k = 0; k < 3; ++k) near_ring[count][k] = a[k];
+        count += 1;
+      } else {
+        was_clipped = 1;
+      }
+      if ((da > 0.0 && db < 0.0) || (da < 0.0 && db > 0.0)) {
+        double t = da / (da - db);
+        for (int k = 0; k < 3; ++k) near_ring[count][k] = a[k] + t * (b[k] - a[k]);
+        count += 1;
+      }
+    }
+    for (int j = 0; j < count; ++j)
+      for (int k = 0; k < 3; ++k) r[j][k] = near_ring[j][k];
+  }
+  Ring8 p;
+  p.n = 0;
+  double depth[SRCV];
+  for (int j = 0; j < count; ++j) {
+    double invz = 1.0;
+    if (persp) invz = 1.0 / r[j][2];
+    double2 x = make_double2((r[j][0] * invz - s[0]) / (s[2] - s[0]),
+                             (r[j][1] * invz - s[1]) / (s[3] - s[1]));
+    ring_append(p, x);
+    depth[j] = persp ? -invz : r[j][2];
+    if (x.x < 0.0 || x.x > 1.0 || x.y < 0.0 || x.y > 1.0) was_clipped = 1;
+  }
+  double4 score = make_double4(0, 0, 0, 0);
+  double det = 0.0;
+  int index = 2;
+  if (count >= 3) {
+    for (int j = 2; j < count; ++j) {
+      det = cross3(p.v[0], p.v[1], p.v[j]);
+      if (det != 0.0) {
+        index = j;
+        break;
+      }
+    }
+  }
+  int face = 0;
+  if (det != 0.0) {
+    double2 pa = make_double2(p.v[1].x - p.v[0].x, p.v[1].y - p.v[0].y);
+    double2 pb = make_double2(p.v[index].x - p.v[0].x, p.v[index].y - p.v[0].y);
+    double s0 = ((depth[1] - depth[0]) * pb.y - (depth[index] - depth[0]) * pa.y) / det;
+    double s1 = (pa.x * (depth[index] - depth[0]) - pb.x * (depth[1] - depth[0])) / det;
+    score = make_double4(s0, s1, depth[0] - s0 * p.v[0].x - s1 * p.v[0].y, 0.0);
+    face = 1;
+    if (det < 0.0) {
+      face = -1;
+      Ring8 q;
+      q.n = 0;
+      for (int j = 0; j < p.n; ++j) ring_append(q, p.v[p.n - 1 - j]);
+      p = q;
+    }
+    ring_keep(p, 1.0, 0.0, 0.0);
+    ring_keep(p, -1.0, 0.0, 1.0);
+    ring_keep(p, 0.0, 1.0, 0.0);
+    ring_keep(p, 0.0, -1.0, 1.0);
+  } else {
+    p.n = 0;
+  }
+  if (!isfinite(score.x) || !isfinite(score.y) || !isfinite(score.z) || fabs(score.x) > 1e28 ||
+      fabs(score.y) > 1e28 || fabs(score.z) > 1e28)
+    set_error(w.status, ERR_INPUT);
+  if (p.n > SRCV) {
+    set_error(w.status, ERR_POLY);
+    p.n = 0;
+  }
+  w.facing[i] = face;
+  w.clipped[i] = was_clipped;
+  w.scores[i] = score;
+  w.src_n[i] = p.n;
+  double lx = 1e30, ly = 1e30, lz = 1e30, hx = -1e30, hy = -1e30, hz = -1e30;
+  for (int j = 0; j < p.n; ++j) {
+    double2 x = p.v[j];
+    w.src_xy[j * nf + i] = x;
+    double z = value(score, x);
+    lx = fmin(lx, x.x), ly = fmin(ly, x.y), lz = fmin(lz, z);
+    hx = fmax(hx, x.x), hy = fmax(hy, x.y), hz = fmax(hz, z);
+  }
+  Box b;
+  pad_box(lx, ly, lz, hx, hy, hz, b.lo, b.hi);
+  w.src_boxes[i] = b;
+}
+
+// ---------------------------------------------------------------------------
+// Undirected edge incidences (warp_peel/kernels.py:471-520): an edge is
+// internal iff exactly two incidences with different start vertices and equal
+// facing; everything else is a contour.
+__device__ __forceinline__ unsigned long long edge_key(const Ws &w, int e) {
+  int f = e / 3, j = e % 3;
+  int a = w.canonical[w.faces[3 * f + j]];
+  int b = w.canonical[w.faces[3 * f + (j + 1) % 3]];
+  return ((unsigned long long)(uint32_t)min(a, b) << 32) | (uint32_t)max(a, b);
+}
+
+__global__ void k_edge_insert(Ws w) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * w.nf || w.status[ST_ERR] != 0) return;
+  unsigned long long key = edge_key(w, e);
+  int mask = w.ehash_size - 1;
+  int slot = (int)(mix64(key) & (uint64_t)mask);
+  for (int probe = 0; probe < w.ehash_size; ++probe) {
+    unsigned long long cur = atomicCAS(&w.ekeys[slot], ~0ull, key);
+    if (cur == ~0ull || cur == key) {
+      int c = atomicAdd(&w.ecount[slot], 1);
+      if (c == 0) w.efirst[slot] = e;
+      if (c == 1) w.esecond[slot] = e;
+      return;
+    }
+    slot = (slot + 1) & mask;
+  }
+}
+
+__global__ void k_edge_mark(Ws w) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * w.nf) return;
+  if (w.status[ST_ERR] != 0) {
+    w.masks[e] = 0;
+    return;
+  }
+  unsigned long long key = edge_key(w, e);
+  int mask = w.ehash_size - 1;
+  int slot = (int)(mix64(key) & (uint64_t)mask);
+  int contour = 1;
+  for (int probe = 0; probe < w.ehash_size; ++probe) {
+    unsigned long long cur = w.ekeys[slot];
+    if (cur == key) {
+      if (w.ecount[slot] == 2) {
+        int e0 = w.efirst[slot], e1 = w.esecond[slot];
+        int a = w.canonical[w.faces[3 * (e0 / 3) + e0 % 3]];
+        int b = w.canonical[w.faces[3 * (e1 / 3) + e1 % 3]];
+        if (a != b && w.facing[e0 / 3] == w.facing[e1 / 3]) contour = 0;
+      }
+      break;
+    }
+    if (cur == ~0ull) break;
+    slot = (slot + 1) & mask;
+  }
+  w.masks[e] = contour;
+}
+
+__device__ __forceinline__ bool contour_edge(const Ws &w, int source, int edge) {
+  // warp_peel/kernels.py:523-536
+  if (w.clipped[source] != 0) return true;
+  int j = edge;
+  if (w.facing[source] < 0) j = (4 - edge) % 3;
+  return w.masks[3 * source + j] != 0;
+}
+
+// ---------------------------------------------------------------------------
+// Contour segments (warp_peel/event_kernels.py:388-501): one slot per source
+// edge, canonical endpoint order, exact duplicate grouping with the minimum
+// depth at each endpoint.
+__device__ __forceinline__ uint64_t segment_hash(double2 a, double2 b) {
+  return mix64(dbits(a.x) ^ mix64(dbits(a.y) + 0x9e3779b97f4a7c15ull) ^
+               mix64(dbits(b.x) + 0x3c6ef372fe94f82aull) ^ mix64(dbits(b.y) + 0xdaa66d2c7ddf743full));
+}
+
+__global__ void k_segments(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nseg) return;
+  int source = i / 8, edge = i % 8;
+  double2 a = make_double2(1e30, 1e30), b = a;
+  bool active = false;
+  if (source < w.nf && w.status[ST_ERR] == 0) {
+    int n = w.src_n[source];
+    if (edge < n && contour_edge(w, source, edge)) {
+      a = w.src_xy[edge * w.nf + source];
+      b = w.src_xy[((edge + 1) % n) * w.nf + source];
+      if (lexless(b, a)) {
+        double2 t = a;
+        a = b;
+        b = t;
+      }
+      active = true;
+    }
+  }
+  w.seg_a[i] = a;
+  w.seg_b[i] = b;
+  w.seg_rep[i] = -1;
+  if (!active) return;
+  // publish the endpoints before the slot index becomes visible in the table
+  __threadfence();
+  int mask = w.shash_size - 1;
+  int slot = (int)(segment_hash(a, b) & (uint64_t)mask);
+  for (int probe = 0; probe < w.shash_size; ++probe) {
+    int cur = atomicCAS(&w.shash[slot], -1, i);
+    if (cur < 0) return;
+    __threadfence();  // acquire: the occupant's endpoints, not a stale L1 copy
+    double2 ca = w.seg_a[cur], cb = w.seg_b[cur];
+    if (same(ca, a) && same(cb, b)) {
+      atomicMin(&w.shash[slot], i);
+      return;
+    }
+    slot = (slot + 1) & mask;
+  }
+}
+
+__global__ void k_segments_reduce(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nseg) return;
+  double2 a = w.seg_a[i], b = w.seg_b[i];
+  if (a.x >= 1e29) return;
+  int mask = w.shash_size - 1;
+  int slot = (int)(segment_hash(a, b) & (uint64_t)mask);
+  int rep = i;
+  for (int probe = 0; probe < w.shash_size; ++probe) {
+    int cur = w.shash[slot];
+    if (cur < 0) break;
+    if (same(w.seg_a[cur], a) && same(w.seg_b[cur], b)) {
+      rep = cur;
+      break;
+    }
+    slot = (slot + 1) & mask;
+  }
+  w.seg_rep[i] = rep;
+  int source = i / 8;
+  double4 sc = w.scores[source];
+  atomic_min_key(&w.seg_near[rep], value(sc, a));
+  atomic_min_key(&w.seg_near[w.nseg + rep], value(sc, b));
+  if (rep == i) atomicAdd((unsigned long long *)&w.metrics[M_UNIQUE], 1ull);
+}
+
+__global__ void k_segment_boxes(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nseg) return;
+  Box b;
+  b.lo = make_float4(1e30f, 1e30f, 1e30f, 0.f);
+  b.hi = make_float4(-1e30f, -1e30f, -1e30f, 0.f);
+  if (w.seg_rep[i] == i) {
+    double2 a = w.seg_a[i], c = w.seg_b[i];
+    double za = unkey(w.seg_near[i]), zc = unkey(w.seg_near[w.nseg + i]);
+    pad_box(fmin(a.x, c.x), fmin(a.y, c.y), fmin(za, zc), fmax(a.x, c.x), fmax(a.y, c.y),
+            fmax(za, zc), b.lo, b.hi);
+  }
+  w.seg_boxes[i] = b;
+}
+
+static inline dim3 grid_for(long n, int block = 256) {
+  long g = (n + block - 1) / block;
+  return dim3((unsigned)(g < 1 ? 1 : g));
+}
+
+void enqueue_prepare(const Ws &w, hipStream_t s) {
+  long reset_n = w.vhash_size;
+  if (w.ehash_size > reset_n) reset_n = w.ehash_size;
+  if (w.shash_size > reset_n) reset_n = w.shash_size;
+  if (2L * w.nseg > reset_n) reset_n = 2L * w.nseg;
+  long reset_grid = (reset_n + 255) / 256;
+  if (reset_grid > 2048) reset_grid = 2048;
+  hipLaunchKernelGGL(k_reset, dim3((unsigned)reset_grid), dim3(256), 0, s, w);
+  long nvf = w.nv > w.nf ? w.nv : w.nf;
+  hipLaunchKernelGGL(k_validate, grid_for(nvf < 1 ? 1 : nvf), dim3(256), 0, s, w);
+  if (w.nv > 0) {
+    hipLaunchKernelGGL(k_weld_insert, grid_for(w.nv), dim3(256), 0, s, w);
+    hipLaunchKernelGGL(k_weld_resolve, grid_for(w.nv), dim3(256), 0, s, w);
+  }
+  if (w.nf > 0) {
+    hipLaunchKernelGGL(k_project, grid_for(w.nf), dim3(256), 0, s, w);
+    hipLaunchKernelGGL(k_edge_insert, grid_for(3L * w.nf), dim3(256), 0, s, w);
+    hipLaunchKernelGGL(k_edge_mark, grid_for(3L * w.nf), dim3(256), 0, s, w);
+  }
+  hipLaunchKernelGGL(k_segments, grid_for(w.nseg), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_segments_reduce, grid_for(w.nseg), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_segment_boxes, grid_for(w.nseg), dim3(256), 0, s, w);
+  Hierarchy st = w.source_tree, ct = w.contour_tree;
+  build_hierarchy(s, w.src_boxes, nullptr, w.nf, w.nf, false, st, w.sort);
+  build_hierarchy(s, w.seg_boxes, nullptr, w.nseg, w.nseg, false, ct, w.sort);
+}
+
+}  // namespace peel

@@ -1,0 +1,52 @@
+// Device-wide primitives: LSD radix sort, exclusive scan, Morton hierarchy.
+// All of them take their element count from device memory, so they can be
+// recorded once into a graph and replayed with changing counts.
+#pragma once
+#include "peel_common.hpp"
+
+namespace peel {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per block tile
+constexpr int RS_MAX_PASSES = 16;
+
+struct SortScratch {
+  int *hist = nullptr;   // [256 * max_blocks] digit-major counts / offsets
+  int *flags = nullptr;  // [RS_MAX_PASSES + 1] non-trivial pass flags
+  int max_blocks = 0;
+};
+
+// Sort (keys, ids) in place by the key bits [bit_begin, bit_end), stably.
+// count: device pointer (clamped to capacity) or nullptr to use count_static.
+// alt_keys / alt_ids are ping-pong buffers of the same capacity.
+void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int *alt_ids,
+                const int *count, int count_static, int capacity, int bit_begin, int bit_end,
+                const SortScratch &scratch);
+
+struct ScanScratch {
+  int *block_sums = nullptr;  // [max_blocks]
+  int max_blocks = 0;
+};
+
+// Exclusive prefix sum of in[0, n) into out[0, n); total written to *total.
+void scan_exclusive(hipStream_t s, const int *in, int *out, const int *count, int count_static,
+                    int capacity, int *total, const ScanScratch &scratch);
+
+struct Hierarchy {
+  uint64_t *keys = nullptr, *alt_keys = nullptr;
+  int *ids = nullptr, *alt_ids = nullptr;  // [size]
+  Box *nodes = nullptr;                     // [2 * size]
+  int size = 1;
+  __host__ __device__ Tree tree() const { return Tree{nodes, ids, size}; }
+};
+
+// Morton-ordered balanced hierarchy over leaf boxes[0, count) (warp_peel/
+// solver.py:34-89 semantics: 15-bit x/y Morton code of the binary32 box
+// centre, layer in the high word when layered, stable order, empty padding).
+void build_hierarchy(hipStream_t s, const Box *boxes, const int *count, int count_static,
+                     int capacity, bool layered, Hierarchy &h, const SortScratch &scratch);
+
+int pow2_at_least(int n);
+
+}  // namespace peel

@@ -1,0 +1,311 @@
+// Stage 2: contour-guided supporting-line subdivision as one persistent
+// work-queue kernel.
+//
+// Reference semantics: warp_peel/frontier.py:15-245 (seed, clean,
+// front_segment, subdivide, copy_frontier) driven by a CUDA conditional while
+// node (warp_peel/events.py:222-264, 804). HIP graphs have no conditional
+// nodes, so the rounds become a device work queue instead:
+//
+//  * Tasks are slots. Slots [0, nf) are the projected sources themselves
+//    (seeds, read in place from the source rings); every cut allocates two
+//    consecutive frontier slots [nf + k, nf + k + 2).
+//  * A lane claims the next ticket (= slot index, wave-aggregated atomicAdd)
+//    and processes it once the slot's ready flag is published (release/acquire
+//    at agent scope). Per visit it inspects at most 32 contour candidates from
+//    the slot's hierarchy cursor, exactly like one reference round; a visit
+//    without a cut keeps the task and continues from the cursor.
+//  * C_PENDING counts live tasks: seeds start it at nf, a cut adds two and
+//    retires one (a single +1), a finished task subtracts one. It only reaches
+//    zero when no task is live and no cut is in flight, so a lane that holds a
+//    ticket that is not ready and sees zero can exit: the ticket will never be
+//    allocated. Every wave reaches that exit; an error bit also exits all
+//    lanes, and a wall-clock guard turns a stall into error bit 2048.
+//
+// Each task's sequence of cuts depends only on its own ring and cursor, so the
+// set of final patches is the same as the reference's round-synchronous
+// frontier for the same hierarchy (only the output order differs).
+#include "workspace.hpp"
+
+namespace peel {
+
+__global__ void k_frontier_reset(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int j = i; j < w.cap.frontier; j += gridDim.x * blockDim.x) w.fr_ready[j] = 0;
+  if (i == 0) {
+    w.control[C_ALLOC] = 0;
+    w.control[C_HEAD] = 0;
+    w.control[C_PENDING] = w.nf;
+  }
+}
+
+struct Slot {
+  const double2 *xy;  // vertex j at xy[j * stride]
+  int stride;
+};
+
+__device__ __forceinline__ Slot slot_of(const Ws &w, int task) {
+  if (task < w.nf) return Slot{w.src_xy + task, w.nf};
+  return Slot{w.fr_xy + (task - w.nf), w.cap.frontier};
+}
+
+__device__ __forceinline__ double2 vtx(const Slot &s, int j) { return s.xy[(long)j * s.stride]; }
+
+// frontier.py:45-71: in-place cleanup of a frontier slot, returns the new count.
+__device__ int clean_slot(double2 *xy, int stride, int n) {
+  if (n > MAXV) return n;
+  int count = 0;
+  double2 last = make_double2(0, 0);
+  for (int j = 0; j < n; ++j) {
+    double2 x = xy[(long)j * stride];
+    if (count == 0 || dlength(make_double2(x.x - last.x, x.y - last.y)) > EPS * 0.01) {
+      xy[(long)count * stride] = x;
+      last = x;
+      count += 1;
+    }
+  }
+  if (count > 1) {
+    double2 first = xy[0];
+    if (dlength(make_double2(first.x - last.x, first.y - last.y)) <= EPS * 0.01) count -= 1;
+  }
+  if (count < 3) return 0;
+  double2 origin = xy[0];
+  double area2 = 0.0, perimeter = 0.0;
+  for (int j = 0; j < count; ++j) {
+    double2 a = xy[(long)j * stride], b = xy[(long)((j + 1) % count) * stride];
+    area2 += cross3(origin, a, b);
+    perimeter += fabs(b.x - a.x) + fabs(b.y - a.y);
+  }
+  if (area2 <= EPS * perimeter) return 0;
+  return count;
+}
+
+// frontier.py:81-103
+__device__ bool front_segment(const Slot &s, int n, double2 a, double2 b, double za, double zb) {
+  double low = 0.0, high = 1.0;
+  for (int j = 0; j <= n; ++j) {
+    double da = za, db = zb;
+    if (j < n) {
+      double2 u = vtx(s, j), v = vtx(s, (j + 1) % n);
+      da = cross3(u, v, a);
+      db = cross3(u, v, b);
+    }
+    if (da <= 0.0 && db <= 0.0) return false;
+    if (da < 0.0) low = fmax(low, da / (da - db));
+    if (db < 0.0) high = fmin(high, da / (da - db));
+  }
+  return high > low;
+}
+
+__device__ __forceinline__ int append_slot(double2 *xy, int stride, int n, double2 x) {
+  if (n < MAXV) xy[(long)n * stride] = x;
+  return n + 1;
+}
+
+__device__ void save_output(const Ws &w, const Slot &s, int n, int source) {
+  // kernels.py:579-596 with layer 0; ranks are assigned later
+  int o = atomicAdd(w.status + ST_POLYS, 1);
+  if (o < w.cap.regions) {
+    for (int j = 0; j < n; ++j) w.out_xy[(long)j * w.cap.regions + o] = vtx(s, j);
+    w.out_n[o] = n;
+    w.out_src[o] = source;
+    w.out_layer[o] = 0;
+    atomicMax(w.status + ST_LAYERS, 1);
+  } else {
+    set_error(w.status, ERR_OUTPUT);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_subdivide(Ws w) {
+  const int nf = w.nf;
+  const Tree tree = w.contour_tree.tree();
+  int lane = threadIdx.x & 63;
+  uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int task = -1, ticket = -1;
+  int n = 0, source = 0, node = 1, gen = 0;
+  long long visits = 0, tests = 0, splits = 0;
+  int max_gen = 0;
+  uint64_t idle_since = wall_clock64();
+  while (true) {
+    if (load_i(w.status + ST_ERR) != 0) break;
+    if (task < 0) {
+      bool need = ticket < 0;
+      uint64_t mask = __ballot(need);
+      if (mask) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(w.control + C_HEAD, __popcll(mask));
+        base = __shfl(base, leader);
+        if (need) ticket = base + __popcll(mask & lt);
+      }
+      // A ticket past the frontier capacity can never be allocated: it only
+      // waits for the pending count to reach zero.
+      bool ready = ticket < nf || (ticket - nf < w.cap.frontier &&
+                                   load_acquire(w.fr_ready + (ticket - nf)) != 0);
+      if (ready) {
+        task = ticket;
+        ticket = -1;
+        if (task < nf) {
+          n = w.src_n[task];
+          source = task;
+          node = 1;
+          gen = 0;
+        } else {
+          int q = task - nf;
+          n = w.fr_n[q];
+          source = w.fr_src[q];
+          node = w.fr_node[q];
+          gen = w.fr_gen[q];
+        }
+        if (n == 0) {
+          atomicSub(w.control + C_PENDING, 1);
+          task = -1;
+        }
+      }
+    }
+    if (task < 0) {
+      if (load_i(w.control + C_PENDING) == 0) break;
+      if (__ballot(task >= 0) == 0) {
+        __builtin_amdgcn_s_sleep(8);
+        if (wall_clock64() - idle_since > 1000000000ull) {
+          set_error(w.status, ERR_ROUNDS);
+          break;
+        }
+      }
+      continue;
+    }
+    idle_since = wall_clock64();
+    // ---- one visit: at most 32 contour candidates (frontier.py:106-223)
+    visits += 1;
+    if (gen > max_gen) max_gen = gen;
+    if (gen >= w.cap.rounds) {
+      set_error(w.status, ERR_ROUNDS);
+      break;
+    }
+    Slot s = slot_of(w, task);
+    double4 score = w.scores[source];
+    double lx = 1e30, ly = 1e30, lz = 1e30, hx = -1e30, hy = -1e30, hz = -1e30;
+    for (int j = 0; j < n; ++j) {
+      double2 x = vtx(s, j);
+      double z = value(score, x);
+      lx = fmin(lx, x.x), ly = fmin(ly, x.y), lz = fmin(lz, z);
+      hx = fmax(hx, x.x), hy = fmax(hy, x.y), hz = fmax(hz, z);
+    }
+    float4 lo, hi;
+    pad_box(lx, ly, lz, hx, hy, hz, lo, hi);
+    lo.z = -1e30f;
+    bool done = false, cut = false;
+    for (int step = 0; step < 32; ++step) {
+      int c = query_next(tree, lo, hi, node);
+      if (c < 0) {
+        done = true;
+        break;
+      }
+      tests += 1;
+      double2 a = w.seg_a[c], b = w.seg_b[c];
+      bool positive = false, negative = false;
+      for (int j = 0; j < n; ++j) {
+        double side = cross3(a, b, vtx(s, j));
+        positive |= side > 0.0;
+        negative |= side < 0.0;
+      }
+      if (!(positive && negative)) continue;
+      double za = value(score, a) - unkey(w.seg_near[c]);
+      double zb = value(score, b) - unkey(w.seg_near[w.nseg + c]);
+      if (!front_segment(s, n, a, b, za, zb)) continue;
+      // ---- cut by the supporting line of (a, b), relative to a
+      int k = atomicAdd(w.control + C_ALLOC, 2);
+      if (k + 2 > w.cap.frontier) {
+        set_error(w.status, ERR_RESIDUAL);
+        cut = true;
+        break;
+      }
+      int fcap = w.cap.frontier;
+      double2 *L = w.fr_xy + k, *R = w.fr_xy + k + 1;
+      int left = 0, right = 0;
+      double2 d = make_double2(b.x - a.x, b.y - a.y);
+      for (int j = 0; j < n; ++j) {
+        double2 u = vtx(s, j), v = vtx(s, (j + 1) % n);
+        double du = -d.y * (u.x - a.x) + d.x * (u.y - a.y) + 0.0;
+        double dv = -d.y * (v.x - a.x) + d.x * (v.y - a.y) + 0.0;
+        if (du >= 0.0) left = append_slot(L, fcap, left, u);
+        if (du <= 0.0) right = append_slot(R, fcap, right, u);
+        if ((du > 0.0 && dv < 0.0) || (du < 0.0 && dv > 0.0)) {
+          double t = du / (du - dv);
+          double2 x = make_double2(u.x + t * (v.x - u.x), u.y + t * (v.y - u.y));
+          left = append_slot(L, fcap, left, x);
+          right = append_slot(R, fcap, right, x);
+        }
+      }
+      left = clean_slot(L, fcap, left);
+      right = clean_slot(R, fcap, right);
+      if (left > MAXV || right > MAXV) {
+        set_error(w.status, ERR_POLY);
+        cut = true;
+        break;
+      }
+      if (left > 0 && right > 0) {
+        splits += 1;
+      } else {
+        // A collapsed child cancels the cut: carry the parent (frontier.py:200-206).
+        for (int j = 0; j < n; ++j) L[(long)j * fcap] = vtx(s, j);
+        left = n;
+        right = 0;
+      }
+      w.fr_n[k] = left;
+      w.fr_n[k + 1] = right;
+      w.fr_src[k] = source;
+      w.fr_src[k + 1] = source;
+      w.fr_node[k] = node;
+      w.fr_node[k + 1] = node;
+      w.fr_gen[k] = gen + 1;
+      w.fr_gen[k + 1] = gen + 1;
+      __threadfence();
+      store_release(w.fr_ready + k, 1);
+      store_release(w.fr_ready + k + 1, 1);
+      int live = atomicAdd(w.control + C_PENDING, 1) + 1;
+      atomicMax(w.status + ST_PEAK, live);
+      cut = true;
+      break;
+    }
+    if (cut) {
+      task = -1;
+    } else if (done) {
+      save_output(w, s, n, source);
+      atomicSub(w.control + C_PENDING, 1);
+      task = -1;
+    } else {
+      gen += 1;  // cursor continues in the next visit, like a re-enqueued patch
+    }
+  }
+  // one atomic per lane for the counters
+  if (visits) atomicAdd((unsigned long long *)&w.metrics[M_VISITS], (unsigned long long)visits);
+  if (tests) atomicAdd((unsigned long long *)&w.metrics[M_CONTOUR_TESTS], (unsigned long long)tests);
+  if (splits) atomicAdd((unsigned long long *)&w.metrics[M_SPLITS], (unsigned long long)splits);
+  atomicMax((unsigned long long *)&w.metrics[M_ROUNDS], (unsigned long long)(max_gen + 1));
+}
+
+__global__ void k_subdivide_finish(Ws w) {
+  // frontier slots used and the live-count peak, for read_status
+  w.metrics[M_SLOTS] = w.control[C_ALLOC];
+  if (w.nf == 0) w.metrics[M_ROUNDS] = 0;
+}
+
+void enqueue_subdivide(const Ws &w, hipStream_t s) {
+  long grid = ((long)w.cap.frontier + 255) / 256;
+  if (grid > 1024) grid = 1024;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_frontier_reset, dim3((unsigned)grid), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_subdivide, dim3(w.persistent_blocks), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_subdivide_finish, dim3(1), dim3(1), 0, s, w);
+}
+
+}  // namespace peel
+
+namespace peel {
+int subdivide_blocks_per_cu() {
+  int blocks = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_subdivide, 256, 0) != hipSuccess)
+    return 2;
+  return blocks;
+}
+}  // namespace peel

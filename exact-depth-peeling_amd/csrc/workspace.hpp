@@ -1,0 +1,108 @@
+// Device workspace of one extraction: every pool, counter and hierarchy,
+// allocated once at creation. Kernels receive this struct by value (it lives in
+// the kernel-argument segment and is read through the scalar cache).
+#pragma once
+#include "primitives.hpp"
+
+namespace peel {
+
+struct Caps {
+  int frontier;       // frontier slots (seeds + two per accepted split)
+  int regions;        // output polygons
+  int hits;           // shared ray hits
+  int vertices;       // boundary vertices
+  int triangles;      // output triangles
+  int rounds;         // subdivision generations
+  int layers;         // layer capacity
+  int edge_vertices;  // edge-noding candidates (<= 128)
+};
+
+struct Ws {
+  // inputs (caller-owned device arrays)
+  const double *vertices;  // [nv][3]
+  const int *faces;        // [nf][3]
+  const double *camera;    // [4][4] row-major
+  const double *settings;  // [6]
+  int nv, nf;
+  Caps cap;
+  int limit, all_layers;
+  int share_regions, propagate_ranks;
+
+  // status / metrics / control
+  int *status;             // [16]
+  long long *metrics;      // [M_COUNT]
+  int *control;            // [C_COUNT]
+  double *inverse;         // [16] inverse camera
+
+  // welding and edge topology
+  int *canonical;          // [nv]
+  int *vhash;              // [vhash_size] vertex ids
+  int vhash_size;
+  unsigned long long *ekeys;  // [ehash_size]
+  int *ecount, *efirst, *esecond;  // [ehash_size]
+  int ehash_size;
+  int *masks;              // [3 nf] 1 = contour edge
+
+  // projected sources: rings [SRCV][nf], count, depth plane, facing, clipped
+  double2 *src_xy;
+  int *src_n;
+  double4 *scores;
+  int *facing, *clipped;
+  Box *src_boxes;          // [nf]
+
+  // contour segments (8 slots per source)
+  int nseg;                // 8 nf
+  double2 *seg_a, *seg_b;  // [nseg]
+  unsigned long long *seg_near;  // [2][nseg] order-preserving keys of min depths
+  int *seg_rep;            // [nseg] representative slot (min slot of duplicates)
+  int *shash;              // [shash_size]
+  int shash_size;
+  Box *seg_boxes;          // [nseg]
+
+  // hierarchies
+  Hierarchy source_tree, contour_tree, point_tree;
+
+  // frontier pool [MAXV][frontier]
+  double2 *fr_xy;
+  int *fr_n, *fr_src, *fr_node, *fr_gen, *fr_ready;
+
+  // output polygons [MAXV][regions]
+  double2 *out_xy;
+  int *out_n, *out_src, *out_layer;
+
+  // ranks
+  double2 *samples;        // [regions]
+  int *reps;               // [regions]
+  int *qhash;              // [qhash_size]
+  int qhash_size;
+  int *groups;             // [regions]
+  int *group_counts;       // [regions]
+  int *group_offsets;      // [regions]
+  int2 *heavy;             // [regions]
+  double *hit_depth;       // [hits]
+  int *hit_src, *hit_group;  // [hits]
+  uint64_t *hkeys, *hkeys_alt;  // [hits]
+  int *hids, *hids_alt;         // [hits]
+
+  // meshing
+  int *vcounts, *voffsets;  // [regions]
+  double2 *bverts;          // [vertices]
+  int *blayers;             // [vertices]
+  int2 *tasks;              // [vertices]
+  Box *point_boxes;         // [vertices]
+  double *tri_uv;           // [triangles][3][2]
+  double *tri_world;        // [triangles][3][3]
+  int *tri_src, *tri_layer;
+
+  SortScratch sort;
+  ScanScratch scan;
+  int persistent_blocks;    // grid of the persistent kernels
+};
+
+// Stage launchers (one translation unit each).
+void enqueue_prepare(const Ws &w, hipStream_t s);
+void enqueue_subdivide(const Ws &w, hipStream_t s);
+void enqueue_rank(const Ws &w, hipStream_t s);
+void enqueue_mesh(const Ws &w, hipStream_t s);
+
+}  // namespace peel

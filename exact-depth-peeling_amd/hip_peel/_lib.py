@@ -1,0 +1,109 @@
+"""ctypes binding of libpeel355.so (declarations: include/peel355.h).
+
+The HIP library is the only compute path: if it is missing or fails to load,
+every constructor raises. There is no CPU fallback.
+"""
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libpeel355.so"
+
+ROUTE_EVENTS, ROUTE_LAYERED = 0, 1
+SHARE_REGIONS, PROPAGATE_RANKS, MERGE_REGIONS, FILTER_FP32 = 1, 2, 4, 8
+MAX_VERTICES = 32
+METRICS_COUNT = 24
+
+
+class PeelDesc(ctypes.Structure):
+    _fields_ = [
+        ("vertices", ctypes.c_void_p),
+        ("num_vertices", ctypes.c_int64),
+        ("faces", ctypes.c_void_p),
+        ("num_faces", ctypes.c_int64),
+        ("camera", ctypes.c_void_p),
+        ("settings", ctypes.c_void_p),
+        ("frontier", ctypes.c_int32),
+        ("regions", ctypes.c_int32),
+        ("hits", ctypes.c_int32),
+        ("vertices_cap", ctypes.c_int32),
+        ("triangles", ctypes.c_int32),
+        ("rounds", ctypes.c_int32),
+        ("layer_capacity", ctypes.c_int32),
+        ("edge_vertices", ctypes.c_int32),
+        ("layers", ctypes.c_int32),
+        ("route", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
+    ]
+
+
+class PeelBuffers(ctypes.Structure):
+    _fields_ = [
+        ("output_xy", ctypes.c_void_p),
+        ("output_n", ctypes.c_void_p),
+        ("output_sources", ctypes.c_void_p),
+        ("output_layers", ctypes.c_void_p),
+        ("regions", ctypes.c_int32),
+        ("max_vertices", ctypes.c_int32),
+        ("triangles_uv", ctypes.c_void_p),
+        ("triangles_world", ctypes.c_void_p),
+        ("triangle_sources", ctypes.c_void_p),
+        ("triangle_layers", ctypes.c_void_p),
+        ("triangles", ctypes.c_int32),
+        ("status", ctypes.c_void_p),
+        ("metrics", ctypes.c_void_p),
+        ("metrics_count", ctypes.c_int32),
+        ("canonical", ctypes.c_void_p),
+        ("stream", ctypes.c_void_p),
+    ]
+
+
+EXPORTS = (
+    "peel_create",
+    "peel_warmup",
+    "peel_enqueue",
+    "peel_capture",
+    "peel_launch",
+    "peel_get_buffers",
+    "peel_read_status",
+    "peel_synchronize",
+    "peel_destroy",
+    "peel_last_error",
+    "peel_version",
+)
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raise if it is absent (no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python exact-depth-peeling_amd/build.py`"
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp = ctypes.c_void_p
+    lib.peel_create.argtypes = [ctypes.POINTER(PeelDesc), ctypes.POINTER(vp)]
+    for name in ("peel_warmup", "peel_enqueue", "peel_capture", "peel_launch", "peel_synchronize", "peel_destroy"):
+        getattr(lib, name).argtypes = [vp]
+        getattr(lib, name).restype = ctypes.c_int
+    lib.peel_create.restype = ctypes.c_int
+    lib.peel_get_buffers.argtypes = [vp, ctypes.POINTER(PeelBuffers)]
+    lib.peel_get_buffers.restype = ctypes.c_int
+    lib.peel_read_status.argtypes = [vp, vp, vp]
+    lib.peel_read_status.restype = ctypes.c_int
+    lib.peel_last_error.restype = ctypes.c_char_p
+    lib.peel_version.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code != 0:
+        message = load().peel_last_error().decode()
+        if code > 0:
+            raise ValueError(message)
+        raise RuntimeError(f"HIP error {-code}: {message}")

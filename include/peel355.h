@@ -1,0 +1,129 @@
+/* peel355 — C-ABI of the MI355X (gfx950) analytic depth peeler.
+ *
+ * Plain pointers and sizes only; device pointers come from any allocator
+ * (the Python host layer passes torch tensor data pointers). Every function
+ * returns 0 on success, a negative value for a HIP runtime error and a
+ * positive value for an argument error; peel_last_error() describes the last
+ * failure of the calling thread.
+ *
+ * The entry points replace the host interface of the reference's device
+ * route (alecjacobson/exact-depth-peeling, warp_peel/), as cited per
+ * function. The reference binds its Python classes directly to Warp; the
+ * Python package hip_peel binds these symbols with ctypes.
+ */
+#ifndef PEEL355_H
+#define PEEL355_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct PeelWorkspace PeelWorkspace;
+
+enum PeelRoute {
+  PEEL_ROUTE_EVENTS = 0,   /* EventPeeler: warp_peel/events.py:36-845 */
+  PEEL_ROUTE_LAYERED = 1   /* DevicePeeler API: warp_peel/solver.py:92-519 */
+};
+
+enum PeelFlags {
+  PEEL_SHARE_REGIONS = 1,    /* events.py:52 share_regions */
+  PEEL_PROPAGATE_RANKS = 2,  /* events.py:53 propagate_ranks */
+  PEEL_MERGE_REGIONS = 4,    /* events.py:54 merge_regions */
+  PEEL_FILTER_FP32 = 8       /* binary32-filtered predicates, binary64 fallback */
+};
+
+/* Constructor arguments: warp_peel/solver.py:103-147 (DevicePeeler.__init__)
+ * and warp_peel/events.py:43-138 (EventPeeler.__init__, EventCapacities
+ * events.py:19-33). Inputs are device arrays: vertices f64 [V][3], faces i32
+ * [F][3], camera f64 [4][4] row-major affine world-to-camera, settings f64 [6]
+ * = xmin, ymin, xmax, ymax, perspective (0/1), near. The caller owns them;
+ * their values may change between launches, their addresses may not. */
+typedef struct PeelDesc {
+  const double *vertices;
+  int64_t num_vertices;
+  const int32_t *faces;
+  int64_t num_faces;
+  const double *camera;
+  const double *settings;
+  int32_t frontier;       /* frontier slots: seeds + two per contour cut */
+  int32_t regions;        /* output polygons */
+  int32_t hits;           /* shared ray hits */
+  int32_t vertices_cap;   /* boundary vertices */
+  int32_t triangles;      /* output triangles */
+  int32_t rounds;         /* subdivision generations */
+  int32_t layer_capacity; /* maximum layers */
+  int32_t edge_vertices;  /* edge-noding candidates, <= 128 */
+  int32_t layers;         /* 0 = all layers, else prefix length */
+  int32_t route;          /* PeelRoute */
+  int32_t flags;          /* PeelFlags */
+} PeelDesc;
+
+/* Device outputs (warp_peel/README.md:186-189 and solver.py:156-200): valid
+ * prefixes are delimited by status[2] (polygons) and status[3] (triangles).
+ * Polygon rings are vertex-major: vertex j of polygon i is output_xy[2 *
+ * (j * regions + i) + {0, 1}]. Triangles are [T][3][2] uv and [T][3][3] world. */
+typedef struct PeelBuffers {
+  double *output_xy;
+  int32_t *output_n;
+  int32_t *output_sources;
+  int32_t *output_layers;
+  int32_t regions;
+  int32_t max_vertices;
+  double *triangles_uv;
+  double *triangles_world;
+  int32_t *triangle_sources;
+  int32_t *triangle_layers;
+  int32_t triangles;
+  int32_t *status;     /* [16] */
+  int64_t *metrics;    /* [metrics_count] */
+  int32_t metrics_count;
+  int32_t *canonical;  /* [V] welded vertex ids */
+  void *stream;        /* hipStream_t owned by the workspace */
+} PeelBuffers;
+
+/* Allocate every pool and create the workspace stream
+ * (replaces DevicePeeler.__init__ / EventPeeler.__init__). */
+int peel_create(const PeelDesc *desc, PeelWorkspace **out);
+
+/* Run the pipeline once, eagerly, before any capture
+ * (replaces warmup(), solver.py:447-457 and events.py:782-796). */
+int peel_warmup(PeelWorkspace *ws);
+
+/* Record the pipeline on the workspace stream: launches only, no allocation,
+ * synchronisation or readback; valid inside an external hipStreamBeginCapture
+ * on that stream (replaces enqueue(), solver.py:459-469, events.py:798-806). */
+int peel_enqueue(PeelWorkspace *ws);
+
+/* Capture the pipeline into the workspace graph (replaces capture(),
+ * solver.py:471-480). */
+int peel_capture(PeelWorkspace *ws);
+
+/* Asynchronous replay on the workspace stream, capturing first if needed
+ * (replaces run(), solver.py:482-486). */
+int peel_launch(PeelWorkspace *ws);
+
+/* Device pointers and capacities of every output. */
+int peel_get_buffers(PeelWorkspace *ws, PeelBuffers *out);
+
+/* Synchronise the workspace stream and copy status[16] and the metrics to
+ * host memory (replaces read_status(), solver.py:488-519, events.py:808-845;
+ * error decoding stays in the host layer). */
+int peel_read_status(PeelWorkspace *ws, int32_t *status, int64_t *metrics);
+
+/* Synchronise the workspace stream. */
+int peel_synchronize(PeelWorkspace *ws);
+
+/* Release every pool, the graph and the stream. */
+int peel_destroy(PeelWorkspace *ws);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *peel_last_error(void);
+
+/* Library version string. */
+const char *peel_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
