@@ -1,0 +1,191 @@
+"""Procedural benchmark scenes (float64, seeded), host-side numpy.
+
+Each generator returns (vertices (V, 3) f64, faces (F, 3) i32, camera (4, 4) f64,
+settings (6,) f64) in the peelers' input layout:
+
+* ``random_triangles``  256 triangles, vertices i.i.d. uniform in [0, 1]^3,
+  orthographic camera looking along -z.
+* ``torus_knot``        closed (2,3) torus-knot tube, 256 x 128 grid = 65,536
+  faces, orthographic view along a seeded random unit direction.
+* ``stack``             k-th congruent triangle (0,0),(1,0),(0,1) at z = k 2^-20.
+* ``nested_shells``     13 closed noisy UV spheres of radius 1.0 .. 2.2, ~55.5K
+  faces each (~721.5K total), orthographic along z, 26 layers.
+"""
+
+import numpy as np
+
+
+def orthographic_view(vertices, direction, margin=0.05):
+    """World-to-camera rotation whose camera z axis is ``direction`` (det +1)
+    and a window around the projected bounding box."""
+    d = np.asarray(direction, dtype=np.float64)
+    d = d / np.linalg.norm(d)
+    helper = np.array([1.0, 0.0, 0.0]) if abs(d[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    e1 = np.cross(helper, d)
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(d, e1)
+    camera = np.eye(4)
+    camera[0, :3], camera[1, :3], camera[2, :3] = e1, e2, d
+    xy = vertices @ camera[:2, :3].T
+    lo, hi = xy.min(axis=0), xy.max(axis=0)
+    pad = margin * (hi - lo).max()
+    settings = np.array([lo[0] - pad, lo[1] - pad, hi[0] + pad, hi[1] + pad, 0.0, 0.01])
+    return camera, settings
+
+
+def grid_faces(rows, cols, wrap_rows=True, wrap_cols=True):
+    """Two triangles per quad of a (rows x cols) vertex grid, counter-clockwise."""
+    faces = []
+    r_end = rows if wrap_rows else rows - 1
+    c_end = cols if wrap_cols else cols - 1
+    for i in range(r_end):
+        for j in range(c_end):
+            a = i * cols + j
+            b = i * cols + (j + 1) % cols
+            c = ((i + 1) % rows) * cols + (j + 1) % cols
+            d = ((i + 1) % rows) * cols + j
+            faces.append((a, b, c))
+            faces.append((a, c, d))
+    return np.array(faces, dtype=np.int32)
+
+
+def torus_knot(seed=0, p=2, q=3, segments=256, sides=128, radius=0.08, scale=1.0):
+    t = np.linspace(0.0, 2.0 * np.pi, segments, endpoint=False)
+
+    def curve(t):
+        r = 2.0 + np.cos(q * t)
+        return scale * np.stack([r * np.cos(p * t), r * np.sin(p * t), -np.sin(q * t)], axis=-1)
+
+    c = curve(t)
+    h = 1e-5
+    tangent = curve(t + h) - curve(t - h)
+    tangent /= np.linalg.norm(tangent, axis=1)[:, None]
+    # parallel transport of a normal, then remove the holonomy linearly
+    normals = np.zeros_like(c)
+    n0 = np.cross(tangent[0], [0.0, 0.0, 1.0])
+    normals[0] = n0 / np.linalg.norm(n0)
+    for i in range(1, segments):
+        n = normals[i - 1] - np.dot(normals[i - 1], tangent[i]) * tangent[i]
+        normals[i] = n / np.linalg.norm(n)
+    end = normals[-1] - np.dot(normals[-1], tangent[0]) * tangent[0]
+    end /= np.linalg.norm(end)
+    binormal0 = np.cross(tangent[0], normals[0])
+    twist = np.arctan2(np.dot(end, binormal0), np.dot(end, normals[0]))
+    verts = np.zeros((segments, sides, 3))
+    for i in range(segments):
+        b = np.cross(tangent[i], normals[i])
+        angle = -twist * i / segments
+        n = np.cos(angle) * normals[i] + np.sin(angle) * b
+        b = np.cross(tangent[i], n)
+        phi = 2.0 * np.pi * np.arange(sides) / sides
+        verts[i] = c[i] + radius * scale * (np.cos(phi)[:, None] * n + np.sin(phi)[:, None] * b)
+    vertices = verts.reshape(-1, 3)
+    faces = grid_faces(segments, sides)
+    rng = np.random.default_rng(seed)
+    direction = rng.normal(size=3)
+    camera, settings = orthographic_view(vertices, direction)
+    return vertices, faces, camera, settings
+
+
+def stack(n=65536):
+    tri = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    v = np.zeros((n, 3, 3))
+    v[:, :, :2] = tri
+    v[:, :, 2] = (np.arange(n) * 2.0**-20)[:, None]
+    faces = np.arange(3 * n, dtype=np.int32).reshape(-1, 3)
+    settings = np.array([-1 / 16, -1 / 16, 17 / 16, 17 / 16, 0.0, 0.01])
+    return v.reshape(-1, 3), faces, np.eye(4), settings
+
+
+def _value_noise(points, seed, octaves=3, frequency=2.0):
+    """Smooth 3D value noise in [-1, 1] (trilinear interpolation of hashed
+    lattice values, smoothstep weights), summed over octaves."""
+    rng = np.random.default_rng(seed)
+    table = rng.uniform(-1.0, 1.0, size=4096)
+    perm = rng.permutation(4096)
+    total = np.zeros(len(points))
+    amplitude, norm = 1.0, 0.0
+    for o in range(octaves):
+        x = points * frequency * (2.0**o)
+        i = np.floor(x).astype(np.int64)
+        f = x - i
+        f = f * f * (3.0 - 2.0 * f)
+        acc = np.zeros(len(points))
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    h = perm[(i[:, 0] + dx) % 4096]
+                    h = perm[(h + i[:, 1] + dy) % 4096]
+                    h = perm[(h + i[:, 2] + dz) % 4096]
+                    wgt = (
+                        (f[:, 0] if dx else 1 - f[:, 0])
+                        * (f[:, 1] if dy else 1 - f[:, 1])
+                        * (f[:, 2] if dz else 1 - f[:, 2])
+                    )
+                    acc += wgt * table[h]
+        total += amplitude * acc
+        norm += amplitude
+        amplitude *= 0.5
+    return total / norm
+
+
+def uv_sphere(radius, n_lat=126, n_lon=222, amplitude=0.0, seed=0):
+    """Closed UV sphere: 2 n_lon (n_lat - 1) faces; radial value-noise offset."""
+    theta = np.pi * np.arange(1, n_lat) / n_lat
+    phi = 2.0 * np.pi * np.arange(n_lon) / n_lon
+    st, ct = np.sin(theta), np.cos(theta)
+    ring = np.stack(
+        [st[:, None] * np.cos(phi)[None], st[:, None] * np.sin(phi)[None], np.repeat(ct[:, None], n_lon, 1)],
+        axis=-1,
+    ).reshape(-1, 3)
+    dirs = np.concatenate([np.array([[0.0, 0.0, 1.0]]), ring, np.array([[0.0, 0.0, -1.0]])])
+    r = radius + amplitude * _value_noise(dirs * radius, seed)
+    vertices = dirs * r[:, None]
+    rows = n_lat - 1
+    faces = []
+    for j in range(n_lon):  # north cap
+        faces.append((0, 1 + j, 1 + (j + 1) % n_lon))
+    body = grid_faces(rows, n_lon, wrap_rows=False, wrap_cols=True) + 1
+    faces = np.concatenate([np.array(faces, dtype=np.int32), body])
+    south = len(dirs) - 1
+    base = 1 + (rows - 1) * n_lon
+    cap = [(base + j, south, base + (j + 1) % n_lon) for j in range(n_lon)]
+    faces = np.concatenate([faces, np.array(cap, dtype=np.int32)])
+    # orient outward: grid_faces winds (lat, lon) counter-clockwise seen from +r? fix per face
+    a, b, c = vertices[faces[:, 0]], vertices[faces[:, 1]], vertices[faces[:, 2]]
+    outward = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) < 0
+    faces[outward] = faces[outward][:, [0, 2, 1]]
+    return vertices, faces.astype(np.int32)
+
+
+def nested_shells(count=13, r0=1.0, step=0.1, amplitude=0.02, n_lat=126, n_lon=222, seed=0):
+    vs, fs, offset = [], [], 0
+    for k in range(count):
+        v, f = uv_sphere(r0 + step * k, n_lat, n_lon, amplitude, seed + k)
+        vs.append(v)
+        fs.append(f + offset)
+        offset += len(v)
+    vertices = np.concatenate(vs)
+    faces = np.concatenate(fs).astype(np.int32)
+    settings = np.array([-2.3, -2.3, 2.3, 2.3, 0.0, 0.01])
+    return vertices, faces, np.eye(4), settings
+
+
+def random_triangles(n=256, seed=0):
+    rng = np.random.default_rng(seed)
+    vertices = rng.random((3 * n, 3))
+    faces = np.arange(3 * n, dtype=np.int32).reshape(-1, 3)
+    camera = np.diag([1.0, -1.0, -1.0, 1.0])  # look along -z, det +1
+    xy = vertices @ camera[:2, :3].T
+    lo, hi = xy.min(axis=0), xy.max(axis=0)
+    pad = 0.05 * (hi - lo).max()
+    settings = np.array([lo[0] - pad, lo[1] - pad, hi[0] + pad, hi[1] + pad, 0.0, 0.01])
+    return vertices, faces, camera, settings
+
+
+CONFIGS = {
+    "torus-knot-65536": torus_knot,
+    "stack-65536": stack,
+    "nested-shells-721500": nested_shells,
+    "random-256": random_triangles,
+}
