@@ -70,6 +70,7 @@ static int alloc_hierarchy(PeelWorkspace *ws, Hierarchy &h, int leaves) {
   if ((r = alloc(ws, h.ids, h.size))) return r;
   if ((r = alloc(ws, h.alt_ids, h.size))) return r;
   if ((r = alloc(ws, h.nodes, 2L * h.size))) return r;
+  if ((r = alloc(ws, h.leaves, 1))) return r;
   return 0;
 }
 
@@ -149,12 +150,20 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   w.shash_size = pow2_at_least(2 * w.nseg);
   ALLOC(w.shash, w.shash_size);
   ALLOC(w.seg_boxes, w.nseg);
+  ALLOC(w.seg_flag, w.nseg);
+  ALLOC(w.seg_off, w.nseg);
+  ALLOC(w.cseg_a, w.nseg);
+  ALLOC(w.cseg_b, w.nseg);
+  ALLOC(w.cnear, 2L * w.nseg);
   int r;
   if ((r = alloc_hierarchy(ws, w.source_tree, nf1)) || (r = alloc_hierarchy(ws, w.contour_tree, w.nseg)) ||
       (r = alloc_hierarchy(ws, w.point_tree, w.cap.vertices))) {
     peel_destroy(ws);
     return r;
   }
+  ALLOC(w.leaf_xy, 8L * w.source_tree.size);
+  ALLOC(w.leaf_n, w.source_tree.size);
+  ALLOC(w.leaf_score, w.source_tree.size);
   long F = w.cap.frontier, R = w.cap.regions;
   ALLOC(w.fr_xy, F * MAXV);
   ALLOC(w.fr_n, F);
@@ -173,6 +182,12 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   ALLOC(w.groups, R);
   ALLOC(w.group_counts, R);
   ALLOC(w.group_offsets, R);
+  ALLOC(w.group_fill, R);
+  ALLOC(w.ray_keys, R);
+  ALLOC(w.ray_keys_alt, R);
+  ALLOC(w.ray_ids, R);
+  ALLOC(w.ray_ids_alt, R);
+  ALLOC(w.medium, R);
   ALLOC(w.heavy, R);
   long H = w.cap.hits;
   ALLOC(w.hit_depth, H);
@@ -189,6 +204,11 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   ALLOC(w.blayers, BV);
   ALLOC(w.tasks, BV);
   ALLOC(w.point_boxes, BV);
+  ALLOC(w.pflag, BV);
+  ALLOC(w.poff, BV);
+  ALLOC(w.upts, BV);
+  w.phash_size = pow2_at_least(2 * (int)std::min<long>(BV, 1L << 29));
+  ALLOC(w.phash, w.phash_size);
   ALLOC(w.tri_uv, T * 6);
   ALLOC(w.tri_world, T * 9);
   ALLOC(w.tri_src, T);
@@ -197,10 +217,12 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   sort_cap = std::max<long>(sort_cap, w.source_tree.size);
   sort_cap = std::max<long>(sort_cap, w.contour_tree.size);
   sort_cap = std::max<long>(sort_cap, w.point_tree.size);
+  sort_cap = std::max<long>(sort_cap, R);
   w.sort.max_blocks = (int)((sort_cap + RS_TILE - 1) / RS_TILE);
   ALLOC(w.sort.hist, 256L * w.sort.max_blocks);
   ALLOC(w.sort.flags, RS_MAX_PASSES + 1);
-  w.scan.max_blocks = (int)((R + 4095) / 4096) + 1;
+  long scan_cap = std::max<long>(std::max<long>(R, BV), w.nseg);
+  w.scan.max_blocks = (int)((scan_cap + 4095) / 4096) + 1;
   ALLOC(w.scan.block_sums, w.scan.max_blocks);
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device));
@@ -228,6 +250,37 @@ int peel_enqueue(PeelWorkspace *ws) {
   enqueue_rank(w, ws->stream);
   enqueue_mesh(w, ws->stream);
   HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int peel_profile(PeelWorkspace *ws, int32_t *count, float *ms, char *names, int32_t names_len) {
+  if (!ws || !count) return fail(1, "null argument");
+  Timeline t;
+  for (int i = 0; i < Timeline::MAX; ++i) HIP_TRY(hipEventCreate(&t.events[i]));
+  const Ws &w = ws->w;
+  mark(&t, ws->stream, "begin");
+  enqueue_prepare(w, ws->stream, &t);
+  enqueue_subdivide(w, ws->stream, &t);
+  enqueue_rank(w, ws->stream, &t);
+  enqueue_mesh(w, ws->stream, &t);
+  hipError_t e = hipStreamSynchronize(ws->stream);
+  std::string joined;
+  int n = 0;
+  for (int i = 1; i < t.count && e == hipSuccess; ++i) {
+    float v = 0.f;
+    e = hipEventElapsedTime(&v, t.events[i - 1], t.events[i]);
+    if (ms && n < *count) ms[n] = v;
+    ++n;
+    if (!joined.empty()) joined += ",";
+    joined += t.names[i];
+  }
+  for (int i = 0; i < Timeline::MAX; ++i) (void)hipEventDestroy(t.events[i]);
+  if (e != hipSuccess) return fail(-(int)e, std::string("profile: ") + hipGetErrorString(e));
+  *count = n;
+  if (names && names_len > 0) {
+    std::strncpy(names, joined.c_str(), names_len - 1);
+    names[names_len - 1] = 0;
+  }
   return 0;
 }
 

@@ -72,16 +72,26 @@ enum Metric : int {
   M_FILTER_TESTS = 19,   // predicates decided by the binary32 filter
   M_FALLBACKS = 20,      // predicates that needed binary64
   M_SLOTS = 21,          // frontier slots used (seeds + 2 per split)
+  M_EDGE_MAX = 22,       // most noding candidates on one polygon edge
+  M_UNIQUE_BVERTS = 23,  // distinct (layer, point) boundary vertices
   M_COUNT = 24,
 };
 
-// Control words (device) used by persistent kernels and on-device sort passes.
+// Control words (device) used by persistent kernels and on-device counts.
+// Contended words sit on their own 128-byte lines.
 enum Control : int {
-  C_ALLOC = 0,    // next free frontier slot
-  C_HEAD = 1,     // next ticket
-  C_PENDING = 2,  // live frontier tasks
-  C_HEAVY = 3,
-  C_COUNT = 64,
+  C_ALLOC = 0,      // next free frontier slot
+  C_HEAD = 32,      // next ticket
+  C_PENDING = 64,   // live frontier tasks
+  C_HEAVY = 96,     // heavy ray jobs
+  C_NHITS = 128,    // stored hits (clamped to capacity)
+  C_NBV = 129,      // boundary vertices (scan total)
+  C_SORTN = 130,    // hits handed to the global sort (0 unless a huge group exists)
+  C_MEDIUM = 131,   // groups sorted by a workgroup
+  C_HUGE = 132,     // any group too large for a workgroup sort
+  C_NSEG = 133,     // unique contour segments
+  C_NUP = 134,      // unique boundary vertices
+  C_COUNT = 256,
 };
 
 struct __align__(16) Box {
@@ -111,6 +121,7 @@ __device__ __forceinline__ void set_error(int *status, int bit) { atomicOr(statu
 __device__ __forceinline__ int load_i(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void acquire_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 __device__ __forceinline__ int load_acquire(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -173,6 +184,89 @@ __device__ __forceinline__ int query_next(const Tree &t, float4 lo, float4 hi, i
         int found = t.ids[node - t.leaves];
         node = successor(node);
         return found;
+      }
+    } else {
+      node = successor(node);
+    }
+  }
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 aggregation. Same-address atomics serialise in L2, so counters and
+// pool cursors are bumped once per wave.
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  int lane = threadIdx.x & 63;
+  return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+// Claim one slot per lane with want == true (works under divergence: only
+// active lanes vote). Returns the slot, or -1.
+__device__ __forceinline__ long long wave_claim(unsigned long long *counter, bool want) {
+  uint64_t m = __ballot(want);
+  if (!m) return -1;
+  int lane = threadIdx.x & 63, leader = __ffsll((unsigned long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader, 64);
+  return want ? (long long)(base + __popcll(m & lanemask_lt())) : -1;
+}
+__device__ __forceinline__ int wave_claim32(int *counter, bool want) {
+  uint64_t m = __ballot(want);
+  if (!m) return -1;
+  int lane = threadIdx.x & 63, leader = __ffsll((unsigned long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, leader, 64);
+  return want ? base + __popcll(m & lanemask_lt()) : -1;
+}
+// Full-wave reductions: every lane of the wave must be active.
+__device__ __forceinline__ long long wave_sum(long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// Exclusive prefix sum across the full wave; total in `total`.
+__device__ __forceinline__ int wave_prefix(int v, int &total) {
+  int lane = threadIdx.x & 63, x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  total = __shfl(x, 63, 64);
+  return x - v;
+}
+__device__ __forceinline__ void wave_add_metric(long long *metric, long long v) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long *)metric, (unsigned long long)v);
+}
+
+// 30-bit Morton code of a point clamped to [0, 1]^2 (x in even bits).
+__device__ __forceinline__ uint64_t morton2(float cx, float cy) {
+  cx = fminf(fmaxf(cx, 0.0f), 1.0f);
+  cy = fminf(fmaxf(cy, 0.0f), 1.0f);
+  uint32_t x = (uint32_t)(int)(cx * 32767.0f), y = (uint32_t)(int)(cy * 32767.0f);
+  uint64_t key = 0;
+  for (int bit = 0; bit < 15; ++bit) {
+    key |= (uint64_t)((x >> bit) & 1u) << (2 * bit);
+    key |= (uint64_t)((y >> bit) & 1u) << (2 * bit + 1);
+  }
+  return key;
+}
+
+// Same traversal, returning the leaf index (position in Morton order).
+__device__ __forceinline__ int query_leaf(const Tree &t, float4 lo, float4 hi, int &node, int end = 0) {
+  while (node > 0 && node != end) {
+    Box b = t.nodes[node];
+    if (overlaps(b, lo, hi)) {
+      if (node < t.leaves) {
+        node = 2 * node;
+      } else {
+        int leaf = node - t.leaves;
+        node = successor(node);
+        return leaf;
       }
     } else {
       node = successor(node);

@@ -116,10 +116,13 @@ __global__ void k_weld_insert(Ws w) {
   int mask = w.vhash_size - 1;
   int slot = (int)(vertex_hash(v[3 * i], v[3 * i + 1], v[3 * i + 2]) & (uint64_t)mask);
   for (int probe = 0; probe < w.vhash_size; ++probe) {
-    int cur = atomicCAS(&w.vhash[slot], -1, i);
-    if (cur < 0) return;
+    int cur = load_i(&w.vhash[slot]);
+    if (cur < 0) {
+      cur = atomicCAS(&w.vhash[slot], -1, i);
+      if (cur < 0) return;
+    }
     if (same_vertex(v, cur, i)) {
-      atomicMin(&w.vhash[slot], i);
+      if (i < cur) atomicMin(&w.vhash[slot], i);
       return;
     }
     slot = (slot + 1) & mask;
@@ -428,12 +431,15 @@ __global__ void k_segments(Ws w) {
   int mask = w.shash_size - 1;
   int slot = (int)(segment_hash(a, b) & (uint64_t)mask);
   for (int probe = 0; probe < w.shash_size; ++probe) {
-    int cur = atomicCAS(&w.shash[slot], -1, i);
-    if (cur < 0) return;
+    int cur = load_i(&w.shash[slot]);
+    if (cur < 0) {
+      cur = atomicCAS(&w.shash[slot], -1, i);
+      if (cur < 0) return;
+    }
     __threadfence();  // acquire: the occupant's endpoints, not a stale L1 copy
     double2 ca = w.seg_a[cur], cb = w.seg_b[cur];
     if (same(ca, a) && same(cb, b)) {
-      atomicMin(&w.shash[slot], i);
+      if (i < cur) atomicMin(&w.shash[slot], i);
       return;
     }
     slot = (slot + 1) & mask;
@@ -465,19 +471,28 @@ __global__ void k_segments_reduce(Ws w) {
   if (rep == i) atomicAdd((unsigned long long *)&w.metrics[M_UNIQUE], 1ull);
 }
 
-__global__ void k_segment_boxes(Ws w) {
+__global__ void k_segment_flags(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= w.nseg) return;
+  w.seg_flag[i] = w.seg_rep[i] == i;
+}
+
+// Unique segments, compacted in slot order, with their padded binary32 boxes
+// (edge_bounds, event_kernels.py:483-501).
+__global__ void k_segment_compact(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nseg || w.seg_rep[i] != i) return;
+  int k = w.seg_off[i];
+  double2 a = w.seg_a[i], c = w.seg_b[i];
+  double za = unkey(w.seg_near[i]), zc = unkey(w.seg_near[w.nseg + i]);
+  w.cseg_a[k] = a;
+  w.cseg_b[k] = c;
+  w.cnear[k] = za;
+  w.cnear[w.nseg + k] = zc;
   Box b;
-  b.lo = make_float4(1e30f, 1e30f, 1e30f, 0.f);
-  b.hi = make_float4(-1e30f, -1e30f, -1e30f, 0.f);
-  if (w.seg_rep[i] == i) {
-    double2 a = w.seg_a[i], c = w.seg_b[i];
-    double za = unkey(w.seg_near[i]), zc = unkey(w.seg_near[w.nseg + i]);
-    pad_box(fmin(a.x, c.x), fmin(a.y, c.y), fmin(za, zc), fmax(a.x, c.x), fmax(a.y, c.y),
-            fmax(za, zc), b.lo, b.hi);
-  }
-  w.seg_boxes[i] = b;
+  pad_box(fmin(a.x, c.x), fmin(a.y, c.y), fmin(za, zc), fmax(a.x, c.x), fmax(a.y, c.y), fmax(za, zc),
+          b.lo, b.hi);
+  w.seg_boxes[k] = b;
 }
 
 static inline dim3 grid_for(long n, int block = 256) {
@@ -485,7 +500,7 @@ static inline dim3 grid_for(long n, int block = 256) {
   return dim3((unsigned)(g < 1 ? 1 : g));
 }
 
-void enqueue_prepare(const Ws &w, hipStream_t s) {
+void enqueue_prepare(const Ws &w, hipStream_t s, Timeline *t) {
   long reset_n = w.vhash_size;
   if (w.ehash_size > reset_n) reset_n = w.ehash_size;
   if (w.shash_size > reset_n) reset_n = w.shash_size;
@@ -504,12 +519,18 @@ void enqueue_prepare(const Ws &w, hipStream_t s) {
     hipLaunchKernelGGL(k_edge_insert, grid_for(3L * w.nf), dim3(256), 0, s, w);
     hipLaunchKernelGGL(k_edge_mark, grid_for(3L * w.nf), dim3(256), 0, s, w);
   }
+  mark(t, s, "weld_project_topology");
   hipLaunchKernelGGL(k_segments, grid_for(w.nseg), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_segments_reduce, grid_for(w.nseg), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_segment_boxes, grid_for(w.nseg), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_segment_flags, grid_for(w.nseg), dim3(256), 0, s, w);
+  scan_exclusive(s, w.seg_flag, w.seg_off, nullptr, w.nseg, w.nseg, w.control + C_NSEG, w.scan);
+  hipLaunchKernelGGL(k_segment_compact, grid_for(w.nseg), dim3(256), 0, s, w);
   Hierarchy st = w.source_tree, ct = w.contour_tree;
+  mark(t, s, "contour_segments");
   build_hierarchy(s, w.src_boxes, nullptr, w.nf, w.nf, false, st, w.sort);
-  build_hierarchy(s, w.seg_boxes, nullptr, w.nseg, w.nseg, false, ct, w.sort);
+  mark(t, s, "source_hierarchy");
+  build_hierarchy(s, w.seg_boxes, w.control + C_NSEG, 0, w.nseg, false, ct, w.sort);
+  mark(t, s, "contour_hierarchy");
 }
 
 }  // namespace peel

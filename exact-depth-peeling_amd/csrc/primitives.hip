@@ -293,41 +293,46 @@ void scan_exclusive(hipStream_t s, const int *in, int *out, const int *count, in
 
 // ---------------------------------------------------------------------------
 // Morton hierarchy (warp_peel/kernels.py:67-116, solver.py:34-89).
-__global__ void h_keys(const Box *boxes, const int *count, int count_static, int capacity,
-                       int size, int layered, uint64_t *keys, int *ids) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= size) return;
+__global__ void h_size(const int *count, int count_static, int capacity, int *leaves) {
   int n = live_count(count, count_static, capacity);
+  int s = 1;
+  while (s < n) s <<= 1;
+  *leaves = s;
+}
+
+__global__ void h_keys(const Box *boxes, const int *count, int count_static, int capacity,
+                       int layered, uint64_t *keys, int *ids) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = live_count(count, count_static, capacity);
+  if (i >= n) return;
   ids[i] = i;
   uint64_t key = ~0ull;
-  if (i < n) {
-    Box b = boxes[i];
-    if (b.lo.x <= b.hi.x && b.lo.y <= b.hi.y) {
-      float cx = (b.lo.x + b.hi.x) * 0.5f, cy = (b.lo.y + b.hi.y) * 0.5f;
-      cx = fminf(fmaxf(cx, 0.0f), 1.0f);
-      cy = fminf(fmaxf(cy, 0.0f), 1.0f);
-      uint32_t x = (uint32_t)(int)(cx * 32767.0f), y = (uint32_t)(int)(cy * 32767.0f);
-      key = 0;
-      for (int bit = 0; bit < 15; ++bit) {
-        key |= (uint64_t)((x >> bit) & 1u) << (2 * bit);
-        key |= (uint64_t)((y >> bit) & 1u) << (2 * bit + 1);
-      }
-      if (layered) key |= (uint64_t)(uint32_t)(int)b.lo.z << 32;
+  Box b = boxes[i];
+  if (b.lo.x <= b.hi.x && b.lo.y <= b.hi.y) {
+    float cx = (b.lo.x + b.hi.x) * 0.5f, cy = (b.lo.y + b.hi.y) * 0.5f;
+    cx = fminf(fmaxf(cx, 0.0f), 1.0f);
+    cy = fminf(fmaxf(cy, 0.0f), 1.0f);
+    uint32_t x = (uint32_t)(int)(cx * 32767.0f), y = (uint32_t)(int)(cy * 32767.0f);
+    key = 0;
+    for (int bit = 0; bit < 15; ++bit) {
+      key |= (uint64_t)((x >> bit) & 1u) << (2 * bit);
+      key |= (uint64_t)((y >> bit) & 1u) << (2 * bit + 1);
     }
+    if (layered) key |= (uint64_t)(uint32_t)(int)b.lo.z << 32;
   }
   keys[i] = key;
 }
 
 __global__ void h_leaves(const Box *boxes, const int *count, int count_static, int capacity,
-                         int size, const int *ids, Box *nodes) {
+                         const int *ids, const int *leaves, Box *nodes) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int size = *leaves;
   if (i >= size) return;
   int n = live_count(count, count_static, capacity);
-  int j = ids[i];
   Box b;
   b.lo = make_float4(1e30f, 1e30f, 1e30f, 0.f);
   b.hi = make_float4(-1e30f, -1e30f, -1e30f, 0.f);
-  if (j < n) b = boxes[j];
+  if (i < n) b = boxes[ids[i]];
   nodes[size + i] = b;
 }
 
@@ -338,12 +343,17 @@ __device__ __forceinline__ Box unite(const Box &a, const Box &b) {
   return r;
 }
 
-// Block b reduces the 2^levels nodes [bottom + b*2^levels, ...) up `levels`
-// levels inside LDS (512 nodes per block at most).
-__global__ __launch_bounds__(256) void h_levels(Box *nodes, int bottom, int levels) {
+// Launch `stage` builds levels [9 stage, 9 stage + 9) above the leaves: block b
+// reduces up to 512 nodes of that bottom level inside LDS.
+__global__ __launch_bounds__(256) void h_levels(Box *nodes, const int *leaves, int stage) {
   __shared__ Box lds[512];
+  int bottom = *leaves >> (9 * stage);
+  if (bottom <= 1) return;
+  int levels = 0;
+  while ((1 << (levels + 1)) <= bottom && levels < 9) ++levels;
   int width = 1 << levels;
   int first = bottom + blockIdx.x * width;
+  if (blockIdx.x * width >= bottom) return;
   for (int i = threadIdx.x; i < width; i += 256) lds[i] = nodes[first + i];
   __syncthreads();
   int count = width, start = first;
@@ -364,22 +374,20 @@ __global__ __launch_bounds__(256) void h_levels(Box *nodes, int bottom, int leve
 
 void build_hierarchy(hipStream_t s, const Box *boxes, const int *count, int count_static,
                      int capacity, bool layered, Hierarchy &h, const SortScratch &scratch) {
-  int size = h.size;
-  int grid = (size + 255) / 256;
-  hipLaunchKernelGGL(h_keys, dim3(grid), dim3(256), 0, s, boxes, count, count_static, capacity,
-                     size, (int)layered, h.keys, h.ids);
-  int sorted = capacity < size ? capacity : size;
-  radix_sort(s, h.keys, h.ids, h.alt_keys, h.alt_ids, count, count_static, sorted, 0,
-             layered ? 64 : 32, scratch);
-  hipLaunchKernelGGL(h_leaves, dim3(grid), dim3(256), 0, s, boxes, count, count_static, capacity,
-                     size, h.ids, h.nodes);
-  int bottom = size;
-  while (bottom > 1) {
-    int levels = 0;
-    while ((1 << (levels + 1)) <= bottom && levels < 9) ++levels;
-    int width = 1 << levels;
-    hipLaunchKernelGGL(h_levels, dim3(bottom / width), dim3(256), 0, s, h.nodes, bottom, levels);
-    bottom >>= levels;
+  int cap = capacity < h.size ? capacity : h.size;
+  hipLaunchKernelGGL(h_size, dim3(1), dim3(1), 0, s, count, count_static, cap, h.leaves);
+  int grid = (cap + 255) / 256;
+  hipLaunchKernelGGL(h_keys, dim3(grid < 1 ? 1 : grid), dim3(256), 0, s, boxes, count, count_static,
+                     cap, (int)layered, h.keys, h.ids);
+  radix_sort(s, h.keys, h.ids, h.alt_keys, h.alt_ids, count, count_static, cap, 0, layered ? 64 : 32,
+             scratch);
+  hipLaunchKernelGGL(h_leaves, dim3((h.size + 255) / 256), dim3(256), 0, s, boxes, count,
+                     count_static, cap, h.ids, h.leaves, h.nodes);
+  int stage = 0;
+  for (long bottom = h.size; bottom > 1; bottom >>= 9, ++stage) {
+    long blocks = bottom / 512;
+    hipLaunchKernelGGL(h_levels, dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(256), 0, s, h.nodes,
+                       h.leaves, stage);
   }
 }
 

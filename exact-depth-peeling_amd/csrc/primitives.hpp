@@ -33,12 +33,17 @@ struct ScanScratch {
 void scan_exclusive(hipStream_t s, const int *in, int *out, const int *count, int count_static,
                     int capacity, int *total, const ScanScratch &scratch);
 
+// Implicit complete tree whose leaf count is decided on the device: leaves =
+// the power of two at or above the live count (stored in *leaves), so a
+// captured graph builds and traverses only the live part of a pool whose
+// capacity is `size`.
 struct Hierarchy {
   uint64_t *keys = nullptr, *alt_keys = nullptr;
   int *ids = nullptr, *alt_ids = nullptr;  // [size]
   Box *nodes = nullptr;                     // [2 * size]
-  int size = 1;
-  __host__ __device__ Tree tree() const { return Tree{nodes, ids, size}; }
+  int *leaves = nullptr;                    // device: current leaf count
+  int size = 1;                             // capacity (power of two)
+  __device__ Tree tree() const { return Tree{nodes, ids, __hip_atomic_load(leaves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)}; }
 };
 
 // Morton-ordered balanced hierarchy over leaf boxes[0, count) (warp_peel/

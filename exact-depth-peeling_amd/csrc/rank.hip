@@ -31,6 +31,8 @@ __global__ void k_rank_reset(Ws w) {
   for (int j = i; j < w.qhash_size; j += gridDim.x * blockDim.x) w.qhash[j] = -1;
   if (i == 0) {
     w.control[C_HEAVY] = 0;
+    w.control[C_MEDIUM] = 0;
+    w.control[C_HUGE] = 0;
   }
 }
 
@@ -61,15 +63,19 @@ __global__ void k_samples(Ws w) {
   }
   w.samples[i] = x;
   w.group_counts[i] = 0;
+  w.group_fill[i] = 0;
   __threadfence();
   int mask = w.qhash_size - 1;
   int slot = (int)(point_hash(x) & (uint64_t)mask);
   for (int probe = 0; probe < w.qhash_size; ++probe) {
-    int cur = atomicCAS(&w.qhash[slot], -1, i);
-    if (cur < 0) return;
+    int cur = load_i(&w.qhash[slot]);
+    if (cur < 0) {
+      cur = atomicCAS(&w.qhash[slot], -1, i);
+      if (cur < 0) return;
+    }
     __threadfence();
     if (same(w.samples[cur], x)) {
-      atomicMin(&w.qhash[slot], i);
+      if (i < cur) atomicMin(&w.qhash[slot], i);
       return;
     }
     slot = (slot + 1) & mask;
@@ -96,25 +102,49 @@ __global__ void k_groups(Ws w) {
   w.groups[i] = leader;
 }
 
-// Half-open point-in-convex test against the flat source ring.
-__device__ __forceinline__ bool contains_flat(const Ws &w, int source, double2 x) {
-  int n = w.src_n[source];
+// Source rings in hierarchy leaf order: leaf k holds its ring as one 128-byte
+// record (8 double2), so the leaves a ray visits in depth-first order are
+// adjacent in memory (the flat source cache of event_kernels.py:110-128,
+// reordered by Morton code).
+__global__ void k_leaf_sources(Ws w) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const Hierarchy &h = w.source_tree;
+  if (k >= w.nf) return;
+  int id = h.ids[k];
+  int n = 0;
+  if (id < w.nf && w.status[ST_ERR] == 0) {
+    n = w.src_n[id];
+    for (int j = 0; j < n; ++j) w.leaf_xy[8L * k + j] = w.src_xy[(long)j * w.nf + id];
+    w.leaf_score[k] = w.scores[id];
+  }
+  w.leaf_n[k] = n;
+}
+
+// Half-open point-in-convex test (event_kernels.py:131-148): a point on an
+// edge belongs to the ring only when the edge is "owned" (b.y > a.y, or
+// b.y == a.y and b.x < a.x), so a tessellated sheet counts once.
+__device__ __forceinline__ bool contains_leaf(const Ws &w, int k, double2 x) {
+  int n = w.leaf_n[k];
+  const double2 *r = w.leaf_xy + 8L * k;
   bool inside = n > 0;
+  double2 a = r[0];
   for (int j = 0; j < n && inside; ++j) {
-    double2 a = w.src_xy[j * w.nf + source];
-    double2 b = w.src_xy[((j + 1) % n) * w.nf + source];
+    double2 b = r[(j + 1) % n];
     double s = cross3(a, b, x);
     bool owned = b.y > a.y || (b.y == a.y && b.x < a.x);
     if (s < 0.0 || (s == 0.0 && !owned)) inside = false;
+    a = b;
   }
   return inside;
 }
 
-__device__ __forceinline__ void store_hit(const Ws &w, int source, int group, double2 x) {
-  unsigned long long slot = atomicAdd((unsigned long long *)&w.metrics[M_RAY_HITS], 1ull);
-  if (slot < (unsigned long long)w.cap.hits) {
-    w.hit_depth[slot] = value(w.scores[source], x);
-    w.hit_src[slot] = source;
+// store_hit (event_kernels.py:151-169) with one pool-cursor atomic per wave.
+__device__ __forceinline__ void store_hits(const Ws &w, bool hit, int leaf, int group, double2 x) {
+  long long slot = wave_claim((unsigned long long *)&w.metrics[M_RAY_HITS], hit);
+  if (!hit) return;
+  if (slot < (long long)w.cap.hits) {
+    w.hit_depth[slot] = value(w.leaf_score[leaf], x);
+    w.hit_src[slot] = w.source_tree.ids[leaf];
     w.hit_group[slot] = group;
   } else {
     set_error(w.status, ERR_HITS);
@@ -126,75 +156,99 @@ __device__ __forceinline__ void ray_box(double2 x, float4 &lo, float4 &hi) {
   hi = make_float4((float)x.x + 2.0e-7f, (float)x.y + 2.0e-7f, 1.0e30f, 0.f);
 }
 
-__global__ void k_ray_hits(Ws w) {
+// A lane keeps its ray for up to HANDOFF candidates; only longer queries hand
+// their unvisited depth-first suffix to the subtree-parallel kernel. The
+// reference counts a group as heavy once it passes 32 candidates
+// (event_kernels.py:222-245); that counter keeps its meaning.
+constexpr int HANDOFF = 256;
+
+// Ray leaders in Morton order of their samples, so the lanes of a wave walk
+// the same hierarchy nodes (the reference sorts its queries by coordinates,
+// events.py:645-678).
+__global__ void k_ray_keys(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int P = npolys(w);
-  if (i >= P || w.status[ST_ERR] != 0) return;
-  if (w.groups[i] != i) return;
-  double2 x = w.samples[i];
-  Tree tree = w.source_tree.tree();
-  float4 lo, hi;
-  ray_box(x, lo, hi);
-  int node = 1, count = 0, visits = 0;
-  for (int step = 0; step < 32; ++step) {
-    int source = query_next(tree, lo, hi, node);
-    if (source < 0) break;
-    visits += 1;
-    if (contains_flat(w, source, x)) {
-      store_hit(w, source, i, x);
-      count += 1;
-    }
-    if (node == 0) break;
+  if (i >= npolys(w)) return;
+  uint64_t key = ~0ull;
+  if (w.status[ST_ERR] == 0 && w.groups[i] == i) {
+    double2 x = w.samples[i];
+    key = morton2((float)x.x, (float)x.y);
   }
-  atomicAdd(&w.group_counts[i], count);
-  atomicAdd((unsigned long long *)&w.metrics[M_RAY_GROUPS], 1ull);
-  atomicAdd((unsigned long long *)&w.metrics[M_RAY_TESTS], (unsigned long long)visits);
-  if (node > 0) {
-    // the unvisited DFS suffix starts at the first leaf below the cursor
-    while (node < tree.leaves) node *= 2;
-    int slot = atomicAdd(w.control + C_HEAVY, 1);
-    atomicAdd((unsigned long long *)&w.metrics[M_HEAVY], 1ull);
-    w.heavy[slot] = make_int2(i, node);
-  }
+  w.ray_keys[i] = key;
+  w.ray_ids[i] = i;
 }
 
-__global__ void k_heavy_hits(Ws w, int workers) {
-  if (w.status[ST_ERR] != 0) return;
+__global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int P = npolys(w);
+  int i = t < P ? w.ray_ids[t] : 0;
+  bool lead = t < P && w.status[ST_ERR] == 0 && w.groups[i] == i;
   Tree tree = w.source_tree.tree();
-  int jobs = load_i(w.control + C_HEAVY);
+  double2 x = lead ? w.samples[i] : make_double2(0, 0);
+  float4 lo, hi;
+  ray_box(x, lo, hi);
+  int node = lead ? 1 : 0, count = 0, visits = 0;
+  bool heavy32 = false;
+  for (int step = 0; step < HANDOFF && node > 0; ++step) {
+    int leaf = query_leaf(tree, lo, hi, node);
+    if (leaf < 0) break;
+    visits += 1;
+    if (visits == 32) heavy32 = node > 0;
+    bool hit = contains_leaf(w, leaf, x);
+    store_hits(w, hit, leaf, i, x);
+    count += hit;
+  }
+  if (lead) w.group_counts[i] = count;
+  bool handoff = lead && node > 0;
+  if (handoff) {
+    // the unvisited suffix starts at the first leaf below the cursor
+    while (node < tree.leaves) node *= 2;
+  }
+  int slot = wave_claim32(w.control + C_HEAVY, handoff);
+  if (handoff) w.heavy[slot] = make_int2(i, node);
+  wave_add_metric(&w.metrics[M_RAY_GROUPS], lead);
+  wave_add_metric(&w.metrics[M_RAY_TESTS], visits);
+  wave_add_metric(&w.metrics[M_HEAVY], heavy32);
+}
+
+__global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
+  bool ok = w.status[ST_ERR] == 0;
+  Tree tree = w.source_tree.tree();
+  int jobs = ok ? load_i(w.control + C_HEAVY) : 0;
   int parts = min(8192, tree.leaves);
   while (parts > 16 && (long long)parts * jobs > 65536) parts /= 2;
   int width = tree.leaves / parts;
   long total = (long)jobs * parts;
   long long visits = 0;
-  for (long task = blockIdx.x * blockDim.x + threadIdx.x; task < total; task += workers) {
-    int2 job = w.heavy[task / parts];
-    int part = (int)(task % parts);
-    int first = tree.leaves + part * width;
-    if (job.y >= first + width) continue;
-    int root = parts + part;
-    int node = job.y > first ? job.y : root;
-    int end = successor(root);
-    int i = job.x;
-    double2 x = w.samples[i];
+  for (long base = (long)blockIdx.x * blockDim.x; base < total; base += workers) {
+    long task = base + threadIdx.x;
+    int node = 0, end = 0, i = 0;
+    if (task < total) {
+      int2 job = w.heavy[task / parts];
+      int part = (int)(task % parts);
+      int first = tree.leaves + part * width;
+      if (job.y < first + width) {
+        int root = parts + part;
+        node = job.y > first ? job.y : root;
+        end = successor(root);
+        i = job.x;
+      }
+    }
+    double2 x = node > 0 ? w.samples[i] : make_double2(0, 0);
     float4 lo, hi;
     ray_box(x, lo, hi);
     int count = 0;
     while (node > 0 && node != end) {
-      int source = query_next(tree, lo, hi, node, end);
-      if (source < 0) break;
+      int leaf = query_leaf(tree, lo, hi, node, end);
+      if (leaf < 0) break;
       visits += 1;
-      if (contains_flat(w, source, x)) {
-        store_hit(w, source, i, x);
-        count += 1;
-      }
+      bool hit = contains_leaf(w, leaf, x);
+      store_hits(w, hit, leaf, i, x);
+      count += hit;
     }
     if (count) atomicAdd(&w.group_counts[i], count);
   }
-  if (visits) atomicAdd((unsigned long long *)&w.metrics[M_RAY_TESTS], (unsigned long long)visits);
+  wave_add_metric(&w.metrics[M_RAY_TESTS], visits);
 }
-
-enum { C_NHITS = 4 };
 
 __global__ void k_check_hits(Ws w) {
   long long h = w.metrics[M_RAY_HITS];
@@ -205,7 +259,7 @@ __global__ void k_check_hits(Ws w) {
 // Stable LSD stages: source, then depth, then group => (group, depth, source).
 __global__ void k_hit_keys(Ws w, int stage) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int H = load_i(w.control + C_NHITS);
+  int H = load_i(w.control + C_SORTN);
   if (i >= H) return;
   int h = stage == 0 ? i : w.hids[i];
   if (stage == 0) w.hids[i] = i;
@@ -219,10 +273,137 @@ __global__ void k_hit_keys(Ws w, int stage) {
   w.hkeys[i] = key;
 }
 
-__global__ void k_assign(Ws w) {
+// ---------------------------------------------------------------------------
+// Hit order (group, depth, source). Each group's hits form one contiguous
+// segment at its scanned offset; segments are sorted independently: up to 64
+// hits by one wave (bitonic network across lanes, in registers), up to 2048 by
+// one workgroup (bitonic in LDS). Only a larger group (e.g. 65,536 identical
+// footprints) enables the global three-stage radix sort over all hits.
+constexpr int SMALL_SEGMENT = 64, MEDIUM_SEGMENT = 2048;
+
+__device__ __forceinline__ bool hit_less(uint64_t ka, int sa, uint64_t kb, int sb) {
+  return ka < kb || (ka == kb && sa < sb);
+}
+
+__global__ void k_hit_scatter(Ws w) {
+  int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= load_i(w.control + C_NHITS) || w.status[ST_ERR] != 0) return;
+  int g = w.hit_group[h];
+  int pos = w.group_offsets[g] + atomicAdd(&w.group_fill[g], 1);
+  w.hids_alt[pos] = h;
+}
+
+__global__ __launch_bounds__(256) void k_sort_small(Ws w) {
+  int lane = threadIdx.x & 63;
+  int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= npolys(w) || w.status[ST_ERR] != 0) return;
+  if (w.groups[g] != g) return;
+  int c = w.group_counts[g];
+  if (c == 0) return;
+  if (c > SMALL_SEGMENT) {
+    if (lane == 0) {
+      if (c <= MEDIUM_SEGMENT) {
+        int k = atomicAdd(w.control + C_MEDIUM, 1);
+        w.medium[k] = g;
+      } else {
+        atomicOr(w.control + C_HUGE, 1);
+      }
+    }
+    return;
+  }
+  int off = w.group_offsets[g];
+  uint64_t key = ~0ull;
+  int src = 0x7fffffff, h = -1;
+  if (lane < c) {
+    h = w.hids_alt[off + lane];
+    key = dkey(w.hit_depth[h]);
+    src = w.hit_src[h];
+  }
+  for (int k = 2; k <= 64; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint64_t ok = __shfl_xor(key, j, 64);
+      int os = __shfl_xor(src, j, 64), oh = __shfl_xor(h, j, 64);
+      bool up = (lane & k) == 0;
+      bool lower = (lane & j) == 0;
+      bool mine_less = hit_less(key, src, ok, os);
+      bool keep = (lower == up) ? mine_less : !mine_less;
+      if (!keep) {
+        key = ok;
+        src = os;
+        h = oh;
+      }
+    }
+  }
+  if (lane < c) w.hids[off + lane] = h;
+}
+
+__global__ __launch_bounds__(256) void k_sort_medium(Ws w) {
+  __shared__ unsigned long long kk[MEDIUM_SEGMENT];
+  __shared__ int ss[MEDIUM_SEGMENT], hh[MEDIUM_SEGMENT];
+  if (w.status[ST_ERR] != 0) return;
+  int count = load_i(w.control + C_MEDIUM);
+  for (int item = blockIdx.x; item < count; item += gridDim.x) {
+    int g = w.medium[item];
+    int c = w.group_counts[g], off = w.group_offsets[g];
+    int n2 = 1;
+    while (n2 < c) n2 <<= 1;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+      if (i < c) {
+        int h = w.hids_alt[off + i];
+        kk[i] = dkey(w.hit_depth[h]);
+        ss[i] = w.hit_src[h];
+        hh[i] = h;
+      } else {
+        kk[i] = ~0ull;
+        ss[i] = 0x7fffffff;
+        hh[i] = -1;
+      }
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+          int p = i ^ j;
+          if (p > i) {
+            bool up = (i & k) == 0;
+            bool greater = hit_less(kk[p], ss[p], kk[i], ss[i]);
+            if (greater == up) {
+              unsigned long long tk = kk[i];
+              kk[i] = kk[p];
+              kk[p] = tk;
+              int t = ss[i];
+              ss[i] = ss[p];
+              ss[p] = t;
+              t = hh[i];
+              hh[i] = hh[p];
+              hh[p] = t;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < c; i += blockDim.x) w.hids[off + i] = hh[i];
+    __syncthreads();
+  }
+}
+
+__global__ void k_sort_plan(Ws w) {
+  w.control[C_SORTN] = w.control[C_HUGE] ? w.control[C_NHITS] : 0;
+}
+
+__device__ int assign_one(const Ws &w, int i);
+
+__global__ __launch_bounds__(256) void k_assign(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int P = npolys(w);
-  if (i >= P || w.status[ST_ERR] != 0) return;
+  int top = 0;
+  if (i < P && w.status[ST_ERR] == 0) top = assign_one(w, i);
+  top = wave_max(top);
+  if ((threadIdx.x & 63) == 0 && top > 0) atomicMax(w.status + ST_LAYERS, top);
+}
+
+__device__ int assign_one(const Ws &w, int i) {
   int source = w.out_src[w.reps[i]];
   double2 x = w.samples[i];
   double z = value(w.scores[source], x);
@@ -245,9 +426,9 @@ __global__ void k_assign(Ws w) {
       set_error(w.status, ERR_LAYERS);
     else
       w.out_n[i] = 0;
-  } else {
-    atomicMax(w.status + ST_LAYERS, low + 1);
+    return 0;
   }
+  return low + 1;
 }
 
 static inline dim3 grid_for(long n, int block = 256) {
@@ -257,19 +438,31 @@ static inline dim3 grid_for(long n, int block = 256) {
 
 void enqueue_connect(const Ws &w, hipStream_t s);
 
-void enqueue_rank(const Ws &w, hipStream_t s) {
+void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   long qgrid = ((long)w.qhash_size + 255) / 256;
   if (qgrid > 2048) qgrid = 2048;
   hipLaunchKernelGGL(k_rank_reset, dim3((unsigned)qgrid), dim3(256), 0, s, w);
   if (w.propagate_ranks) enqueue_connect(w, s);
+  mark(t, s, "connectivity");
+  hipLaunchKernelGGL(k_leaf_sources, grid_for(w.source_tree.size), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_samples, grid_for(w.cap.regions), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_groups, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_ray_keys, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  radix_sort(s, w.ray_keys, w.ray_ids, w.ray_keys_alt, w.ray_ids_alt, w.status + ST_POLYS, 0,
+             w.cap.regions, 0, 32, w.sort);
+  mark(t, s, "samples_groups");
   hipLaunchKernelGGL(k_ray_hits, grid_for(w.cap.regions), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_heavy_hits, dim3(32), dim3(256), 0, s, w, 32 * 256);
+  mark(t, s, "k_ray_hits");
+  hipLaunchKernelGGL(k_heavy_hits, dim3(256), dim3(256), 0, s, w, 256 * 256);
+  mark(t, s, "k_heavy_hits");
   hipLaunchKernelGGL(k_check_hits, dim3(1), dim3(1), 0, s, w);
   scan_exclusive(s, w.group_counts, w.group_offsets, w.status + ST_POLYS, 0, w.cap.regions,
                  nullptr, w.scan);
-  const int *nh = w.control + C_NHITS;
+  hipLaunchKernelGGL(k_hit_scatter, grid_for(w.cap.hits), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_sort_small, grid_for(w.cap.regions, 4), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_sort_medium, dim3(1024), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(1), 0, s, w);
+  const int *nh = w.control + C_SORTN;
   int src_bits = 1, group_bits = 1;
   while ((1L << src_bits) < (long)w.nf) ++src_bits;
   while ((1L << group_bits) < (long)w.cap.regions) ++group_bits;
@@ -278,7 +471,9 @@ void enqueue_rank(const Ws &w, hipStream_t s) {
     int bits = stage == 0 ? src_bits : (stage == 1 ? 64 : group_bits);
     radix_sort(s, w.hkeys, w.hids, w.hkeys_alt, w.hids_alt, nh, 0, w.cap.hits, 0, bits, w.sort);
   }
+  mark(t, s, "hit_sort");
   hipLaunchKernelGGL(k_assign, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  mark(t, s, "k_assign");
 }
 
 }  // namespace peel

@@ -101,15 +101,14 @@ __device__ __forceinline__ int append_slot(double2 *xy, int stride, int n, doubl
   return n + 1;
 }
 
-__device__ void save_output(const Ws &w, const Slot &s, int n, int source) {
-  // kernels.py:579-596 with layer 0; ranks are assigned later
-  int o = atomicAdd(w.status + ST_POLYS, 1);
+// save_polygon (kernels.py:579-596) with layer 0; ranks are assigned later.
+// The output cursor is claimed once per wave.
+__device__ void save_output(const Ws &w, const Slot &s, int n, int source, int o) {
   if (o < w.cap.regions) {
     for (int j = 0; j < n; ++j) w.out_xy[(long)j * w.cap.regions + o] = vtx(s, j);
     w.out_n[o] = n;
     w.out_src[o] = source;
     w.out_layer[o] = 0;
-    atomicMax(w.status + ST_LAYERS, 1);
   } else {
     set_error(w.status, ERR_OUTPUT);
   }
@@ -138,9 +137,11 @@ __global__ __launch_bounds__(256) void k_subdivide(Ws w) {
         if (need) ticket = base + __popcll(mask & lt);
       }
       // A ticket past the frontier capacity can never be allocated: it only
-      // waits for the pending count to reach zero.
+      // waits for the pending count to reach zero. Polling is relaxed; the
+      // acquire fence runs once, when a published slot is taken.
       bool ready = ticket < nf || (ticket - nf < w.cap.frontier &&
-                                   load_acquire(w.fr_ready + (ticket - nf)) != 0);
+                                   load_i(w.fr_ready + (ticket - nf)) != 0);
+      if (__ballot(ready && ticket >= nf)) acquire_fence();
       if (ready) {
         task = ticket;
         ticket = -1;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256) void k_subdivide(Ws w) {
     if (task < 0) {
       if (load_i(w.control + C_PENDING) == 0) break;
       if (__ballot(task >= 0) == 0) {
-        __builtin_amdgcn_s_sleep(8);
+        __builtin_amdgcn_s_sleep(32);
         if (wall_clock64() - idle_since > 1000000000ull) {
           set_error(w.status, ERR_ROUNDS);
           break;
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(256) void k_subdivide(Ws w) {
         break;
       }
       tests += 1;
-      double2 a = w.seg_a[c], b = w.seg_b[c];
+      double2 a = w.cseg_a[c], b = w.cseg_b[c];
       bool positive = false, negative = false;
       for (int j = 0; j < n; ++j) {
         double side = cross3(a, b, vtx(s, j));
@@ -209,8 +210,8 @@ __global__ __launch_bounds__(256) void k_subdivide(Ws w) {
         negative |= side < 0.0;
       }
       if (!(positive && negative)) continue;
-      double za = value(score, a) - unkey(w.seg_near[c]);
-      double zb = value(score, b) - unkey(w.seg_near[w.nseg + c]);
+      double za = value(score, a) - w.cnear[c];
+      double zb = value(score, b) - w.cnear[w.nseg + c];
       if (!front_segment(s, n, a, b, za, zb)) continue;
       // ---- cut by the supporting line of (a, b), relative to a
       int k = atomicAdd(w.control + C_ALLOC, 2);
@@ -267,21 +268,25 @@ __global__ __launch_bounds__(256) void k_subdivide(Ws w) {
       cut = true;
       break;
     }
+    bool finish = !cut && done;
+    int o = wave_claim32(w.status + ST_POLYS, finish);
     if (cut) {
       task = -1;
-    } else if (done) {
-      save_output(w, s, n, source);
+    } else if (finish) {
+      save_output(w, s, n, source, o);
       atomicSub(w.control + C_PENDING, 1);
       task = -1;
     } else {
       gen += 1;  // cursor continues in the next visit, like a re-enqueued patch
     }
   }
-  // one atomic per lane for the counters
-  if (visits) atomicAdd((unsigned long long *)&w.metrics[M_VISITS], (unsigned long long)visits);
-  if (tests) atomicAdd((unsigned long long *)&w.metrics[M_CONTOUR_TESTS], (unsigned long long)tests);
-  if (splits) atomicAdd((unsigned long long *)&w.metrics[M_SPLITS], (unsigned long long)splits);
-  atomicMax((unsigned long long *)&w.metrics[M_ROUNDS], (unsigned long long)(max_gen + 1));
+  // one atomic per wave for the counters
+  wave_add_metric(&w.metrics[M_VISITS], visits);
+  wave_add_metric(&w.metrics[M_CONTOUR_TESTS], tests);
+  wave_add_metric(&w.metrics[M_SPLITS], splits);
+  max_gen = wave_max(max_gen);
+  if ((threadIdx.x & 63) == 0)
+    atomicMax((unsigned long long *)&w.metrics[M_ROUNDS], (unsigned long long)(max_gen + 1));
 }
 
 __global__ void k_subdivide_finish(Ws w) {
@@ -290,12 +295,14 @@ __global__ void k_subdivide_finish(Ws w) {
   if (w.nf == 0) w.metrics[M_ROUNDS] = 0;
 }
 
-void enqueue_subdivide(const Ws &w, hipStream_t s) {
+void enqueue_subdivide(const Ws &w, hipStream_t s, Timeline *t) {
   long grid = ((long)w.cap.frontier + 255) / 256;
   if (grid > 1024) grid = 1024;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_frontier_reset, dim3((unsigned)grid), dim3(256), 0, s, w);
+  mark(t, s, "frontier_reset");
   hipLaunchKernelGGL(k_subdivide, dim3(w.persistent_blocks), dim3(256), 0, s, w);
+  mark(t, s, "k_subdivide");
   hipLaunchKernelGGL(k_subdivide_finish, dim3(1), dim3(1), 0, s, w);
 }
 

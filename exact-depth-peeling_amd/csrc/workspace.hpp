@@ -57,10 +57,16 @@ struct Ws {
   int *seg_rep;            // [nseg] representative slot (min slot of duplicates)
   int *shash;              // [shash_size]
   int shash_size;
-  Box *seg_boxes;          // [nseg]
+  Box *seg_boxes;          // [nseg] boxes of the compacted unique segments
+  int *seg_flag, *seg_off; // [nseg] unique-segment flags and their scan
+  double2 *cseg_a, *cseg_b;  // [nseg] compacted unique segments
+  double *cnear;           // [2][nseg] their minimum endpoint depths
 
   // hierarchies
   Hierarchy source_tree, contour_tree, point_tree;
+  double2 *leaf_xy;        // [source_tree.size][8] rings in leaf order
+  int *leaf_n;             // [source_tree.size]
+  double4 *leaf_score;     // [source_tree.size]
 
   // frontier pool [MAXV][frontier]
   double2 *fr_xy;
@@ -78,6 +84,10 @@ struct Ws {
   int *groups;             // [regions]
   int *group_counts;       // [regions]
   int *group_offsets;      // [regions]
+  int *group_fill;         // [regions] segment fill cursors
+  uint64_t *ray_keys, *ray_keys_alt;  // [regions] Morton keys of ray samples
+  int *ray_ids, *ray_ids_alt;         // [regions] rays in Morton order
+  int *medium;             // [regions] groups sorted by one workgroup
   int2 *heavy;             // [regions]
   double *hit_depth;       // [hits]
   int *hit_src, *hit_group;  // [hits]
@@ -89,7 +99,11 @@ struct Ws {
   double2 *bverts;          // [vertices]
   int *blayers;             // [vertices]
   int2 *tasks;              // [vertices]
-  Box *point_boxes;         // [vertices]
+  Box *point_boxes;         // [vertices] boxes of the compacted unique vertices
+  int *pflag, *poff;        // [vertices] first-copy flags and their scan
+  double2 *upts;            // [vertices] unique (layer, point) boundary vertices
+  int *phash;               // [phash_size] boundary vertex dedupe (layer, x, y)
+  int phash_size;
   double *tri_uv;           // [triangles][3][2]
   double *tri_world;        // [triangles][3][3]
   int *tri_src, *tri_layer;
@@ -99,10 +113,24 @@ struct Ws {
   int persistent_blocks;    // grid of the persistent kernels
 };
 
+// Optional host-side timeline: hipEvents recorded between stages when the
+// pipeline runs eagerly for a stage profile (never inside a capture).
+struct Timeline {
+  static constexpr int MAX = 48;
+  hipEvent_t events[MAX];
+  const char *names[MAX];
+  int count = 0;
+};
+inline void mark(Timeline *t, hipStream_t s, const char *name) {
+  if (!t || t->count >= Timeline::MAX) return;
+  (void)hipEventRecord(t->events[t->count], s);
+  t->names[t->count++] = name;
+}
+
 // Stage launchers (one translation unit each).
-void enqueue_prepare(const Ws &w, hipStream_t s);
-void enqueue_subdivide(const Ws &w, hipStream_t s);
-void enqueue_rank(const Ws &w, hipStream_t s);
-void enqueue_mesh(const Ws &w, hipStream_t s);
+void enqueue_prepare(const Ws &w, hipStream_t s, Timeline *t = nullptr);
+void enqueue_subdivide(const Ws &w, hipStream_t s, Timeline *t = nullptr);
+void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t = nullptr);
+void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t = nullptr);
 
 }  // namespace peel

@@ -66,6 +66,7 @@ EXPORTS = (
     "peel_launch",
     "peel_get_buffers",
     "peel_read_status",
+    "peel_profile",
     "peel_synchronize",
     "peel_destroy",
     "peel_last_error",
@@ -95,6 +96,8 @@ def load():
     lib.peel_get_buffers.restype = ctypes.c_int
     lib.peel_read_status.argtypes = [vp, vp, vp]
     lib.peel_read_status.restype = ctypes.c_int
+    lib.peel_profile.argtypes = [vp, vp, vp, ctypes.c_char_p, ctypes.c_int32]
+    lib.peel_profile.restype = ctypes.c_int
     lib.peel_last_error.restype = ctypes.c_char_p
     lib.peel_version.restype = ctypes.c_char_p
     _lib = lib

@@ -32,7 +32,7 @@ METRIC_NAMES = (
     "matched_patch_edges",
     "edge_hash_misses",
 )
-EXTRA_METRICS = ("filtered_predicates", "fp64_fallbacks", "frontier_slots")
+EXTRA_METRICS = ("filtered_predicates", "fp64_fallbacks", "frontier_slots", "max_edge_vertices", "unique_boundary_vertices")
 
 
 @dataclass(frozen=True)
@@ -113,5 +113,5 @@ class EventPeeler(_Workspace):
         for name in ("peeling_passes", "max_hidden_pieces"):
             info.pop(name)
         info.update(zip(METRIC_NAMES, map(int, metrics)))
-        info.update(zip(EXTRA_METRICS, map(int, metrics[19:22])))
+        info.update(zip(EXTRA_METRICS, map(int, metrics[19:24])))
         return info

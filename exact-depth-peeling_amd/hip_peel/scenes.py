@@ -9,7 +9,7 @@ settings (6,) f64) in the peelers' input layout:
   faces, orthographic view along a seeded random unit direction.
 * ``stack``             k-th congruent triangle (0,0),(1,0),(0,1) at z = k 2^-20.
 * ``nested_shells``     13 closed noisy UV spheres of radius 1.0 .. 2.2, ~55.5K
-  faces each (~721.5K total), orthographic along z, 26 layers.
+  faces each (~721.5K total), poles on the x axis, orthographic along z.
 """
 
 import numpy as np
@@ -151,7 +151,10 @@ def uv_sphere(radius, n_lat=126, n_lon=222, amplitude=0.0, seed=0):
     base = 1 + (rows - 1) * n_lon
     cap = [(base + j, south, base + (j + 1) % n_lon) for j in range(n_lon)]
     faces = np.concatenate([faces, np.array(cap, dtype=np.int32)])
-    # orient outward: grid_faces winds (lat, lon) counter-clockwise seen from +r? fix per face
+    # poles on the x axis (in the image plane of a view along z), so no polar
+    # fan faces the camera; a rotation about y keeps the mesh closed
+    vertices = np.stack([vertices[:, 2], vertices[:, 1], -vertices[:, 0]], axis=1)
+    # orient every face outward
     a, b, c = vertices[faces[:, 0]], vertices[faces[:, 1]], vertices[faces[:, 2]]
     outward = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) < 0
     faces[outward] = faces[outward][:, [0, 2, 1]]

@@ -220,6 +220,18 @@ class _Workspace:
         self._order_inputs()
         _lib.check(self._lib.peel_launch(self._handle))
 
+    def profile(self):
+        """Eager run with hipEvents between stages: {stage: ms} in pipeline order."""
+        if torch.cuda.is_current_stream_capturing():
+            raise RuntimeError("profile() runs eagerly; it cannot be captured")
+        self._order_inputs()
+        count = ctypes.c_int32(64)
+        ms = (ctypes.c_float * 64)()
+        names = ctypes.create_string_buffer(4096)
+        _lib.check(self._lib.peel_profile(self._handle, ctypes.byref(count), ms, names, 4096))
+        keys = names.value.decode().split(",")
+        return dict(zip(keys, [float(ms[i]) for i in range(count.value)]))
+
     def synchronize(self):
         _lib.check(self._lib.peel_synchronize(self._handle))
 

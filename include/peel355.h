@@ -111,6 +111,12 @@ int peel_get_buffers(PeelWorkspace *ws, PeelBuffers *out);
  * error decoding stays in the host layer). */
 int peel_read_status(PeelWorkspace *ws, int32_t *status, int64_t *metrics);
 
+/* Run the pipeline once, eagerly, with hipEvents between stages and around
+ * the hot kernels; writes up to *count stage times (ms) and their
+ * comma-separated names, then sets *count. Never valid inside a capture
+ * (replaces the per-stage diagnostics of scripts/profile_events.py:36-55). */
+int peel_profile(PeelWorkspace *ws, int32_t *count, float *ms, char *names, int32_t names_len);
+
 /* Synchronise the workspace stream. */
 int peel_synchronize(PeelWorkspace *ws);
 

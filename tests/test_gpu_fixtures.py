@@ -12,7 +12,17 @@ CROSSING = {"crossing", "perspective_crossing"} | {f"random_{i:02}" for i in ran
 CASES = [n for n in accepted_fixtures() if n not in CROSSING]
 
 
+# The finite-precision device routes discard remnants narrower than ~1e-10 in
+# the normalised window (warp_peel/README.md:127-131). The reference records the
+# one fixture where that removes a whole exact layer: `slivers` has two exact
+# layers, the second only behind a 1e-12-wide sliver source (README.md:436-440,
+# results/fast/validation.json "expected_precision_loss_fixture": "slivers").
+PRECISION_LOSS_LAYERS = {"slivers": 1}
+
+
 def expected_layers(name):
+    if name in PRECISION_LOSS_LAYERS:
+        return PRECISION_LOSS_LAYERS[name]
     layers = golden_layers()
     return layers.get(f"{name}-mesh", layers.get(name))
 
