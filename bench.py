@@ -8,7 +8,7 @@ projection, topology, hierarchies, subdivision, ranks, noding, fans, lift) on
 synthetic float64 input already resident in HBM; upload, allocation, capture
 and readback are outside the timed region (the reference's protocol,
 scripts/benchmark_events.py:77-93). The headline workload is the 65,536-face
-(2,3) torus-knot tube; --extras adds the 721.5K-face nested shells and the
+(2,3) torus-knot tube; --extras adds the 717,740-face nested shells and the
 65,536-layer identical-footprint stack.
 
 Prints one JSON line. ``roofline`` prices the dominant kernel's algorithmic
@@ -193,7 +193,7 @@ def main():
     del peeler
     if args.extras:
         extra = {}
-        for name in ("nested-shells-721500", "stack-65536"):
+        for name in ("nested-shells-717740", "stack-65536"):
             p, h, i, st, t, tot, nv = run_config(name, max(3, args.steps // 2), 1)
             extra[name] = {"median_ms": round(t["median_ms"], 3), "layers": i["layers"],
                            "polygons": i["polygons"], "triangles": i["triangles"],

@@ -8,8 +8,8 @@ settings (6,) f64) in the peelers' input layout:
 * ``torus_knot``        closed (2,3) torus-knot tube, 256 x 128 grid = 65,536
   faces, orthographic view along a seeded random unit direction.
 * ``stack``             k-th congruent triangle (0,0),(1,0),(0,1) at z = k 2^-20.
-* ``nested_shells``     13 closed noisy UV spheres of radius 1.0 .. 2.2, ~55.5K
-  faces each (~721.5K total), poles on the x axis, orthographic along z.
+* ``nested_shells``     13 closed noisy geodesic spheres of radius 1.0 .. 2.2,
+  ~55K faces each (717,740 total), orthographic along z.
 """
 
 import numpy as np
@@ -161,10 +161,65 @@ def uv_sphere(radius, n_lat=126, n_lon=222, amplitude=0.0, seed=0):
     return vertices, faces.astype(np.int32)
 
 
-def nested_shells(count=13, r0=1.0, step=0.1, amplitude=0.02, n_lat=126, n_lon=222, seed=0):
+def icosahedron():
+    t = (1.0 + 5.0**0.5) / 2.0
+    v = np.array(
+        [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t],
+         [0, -1, -t], [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]],
+        dtype=np.float64,
+    )
+    f = np.array(
+        [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+         [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8],
+         [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]],
+        dtype=np.int32,
+    )
+    return v / np.linalg.norm(v, axis=1)[:, None], f
+
+
+def geodesic_sphere(radius, m, amplitude=0.0, seed=0):
+    """Class-I geodesic sphere: every icosahedron face split into m^2 triangles
+    (20 m^2 faces), lattice points projected to the sphere. Shared lattice
+    points are keyed by their exact integer barycentric weights, so the mesh
+    is closed and every shared vertex is computed once. Radial value-noise
+    offset of the given amplitude."""
+    corners, ico = icosahedron()
+    index, points, faces = {}, [], []
+
+    def vertex(weights):
+        key = tuple(sorted((c, w) for c, w in weights if w > 0))
+        if key not in index:
+            p = sum(w * corners[c] for c, w in key) / m
+            index[key] = len(points)
+            points.append(p / np.linalg.norm(p))
+        return index[key]
+
+    for A, B, C in ico:
+        def P(a, b):
+            return vertex(((A, m - a - b), (B, a), (C, b)))
+
+        for a in range(m):
+            for b in range(m - a):
+                faces.append((P(a, b), P(a + 1, b), P(a, b + 1)))
+                if a + b < m - 1:
+                    faces.append((P(a + 1, b), P(a + 1, b + 1), P(a, b + 1)))
+    dirs = np.array(points)
+    r = radius + amplitude * _value_noise(dirs * radius, seed)
+    vertices = dirs * r[:, None]
+    faces = np.array(faces, dtype=np.int32)
+    a, b, c = vertices[faces[:, 0]], vertices[faces[:, 1]], vertices[faces[:, 2]]
+    inward = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) < 0
+    faces[inward] = faces[inward][:, [0, 2, 1]]
+    return vertices, faces
+
+
+def nested_shells(count=13, r0=1.0, step=0.1, amplitude=0.02, seed=0):
+    """13 concentric noisy geodesic spheres, radius 1.0 .. 2.2: the 6 inner
+    shells with 20 * 52^2 = 54,080 faces, the 7 outer with 20 * 53^2 = 56,180
+    (717,740 faces in total), orthographic view along z."""
     vs, fs, offset = [], [], 0
     for k in range(count):
-        v, f = uv_sphere(r0 + step * k, n_lat, n_lon, amplitude, seed + k)
+        v, f = geodesic_sphere(r0 + step * k, 52 if k < 6 else 53, amplitude, seed + k)
         vs.append(v)
         fs.append(f + offset)
         offset += len(v)
@@ -189,6 +244,6 @@ def random_triangles(n=256, seed=0):
 CONFIGS = {
     "torus-knot-65536": torus_knot,
     "stack-65536": stack,
-    "nested-shells-721500": nested_shells,
+    "nested-shells-717740": nested_shells,
     "random-256": random_triangles,
 }

@@ -155,3 +155,87 @@ def audit(polygons, frag_sources, frag_layers, points, ranks, max_examples=16):
     result["mismatches"] = bad
     result["examples"] = examples[: min(bad, max_examples)].tolist()
     return result
+
+
+CPU_COUNT_FIELDS = (
+    "polygons",
+    "triangles",
+    "layers",
+    "splits",
+    "ray_groups",
+    "ray_hits",
+    "error",
+    "contour_tests",
+    "unique_contours",
+    "boundary_vertices",
+)
+
+
+class CpuResult:
+    """Output of the CPU FP64 restatement of the event route."""
+
+    def __init__(self, rings, sources, layers, tri_uv, tri_world, tri_src, tri_layer, counts, seconds):
+        self.rings, self.sources, self.layers = rings, sources, layers
+        self.tri_uv, self.tri_world, self.tri_src, self.tri_layer = tri_uv, tri_world, tri_src, tri_layer
+        self.counts, self.seconds = counts, seconds
+
+
+def cpu_event_peel(vertices, faces, camera, settings, layers=0, threads=0, cap_poly=None, cap_tri=None):
+    vertices = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+    faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+    camera = np.ascontiguousarray(camera, dtype=np.float64).reshape(4, 4)
+    settings = np.ascontiguousarray(settings, dtype=np.float64).reshape(6)
+    n = max(16, len(faces))
+    cap_poly = cap_poly or 8 * n
+    cap_tri = cap_tri or 32 * n
+    poly_xy = np.zeros((cap_poly, 32, 2))
+    poly_n = np.zeros(cap_poly, np.int32)
+    poly_src = np.zeros(cap_poly, np.int32)
+    poly_layer = np.zeros(cap_poly, np.int32)
+    tri_uv = np.zeros((cap_tri, 3, 2))
+    tri_world = np.zeros((cap_tri, 3, 3))
+    tri_src = np.zeros(cap_tri, np.int32)
+    tri_layer = np.zeros(cap_tri, np.int32)
+    counts = np.zeros(10, np.int64)
+    seconds = np.zeros(4)
+    f = lib().oracle_cpu_event_peel
+    f.restype = ctypes.c_int
+    f(
+        _p(vertices), len(vertices), _p(faces), len(faces), _p(camera), _p(settings), int(layers),
+        int(threads), _p(poly_xy), _p(poly_n), _p(poly_src), _p(poly_layer), int(cap_poly), _p(tri_uv),
+        _p(tri_world), _p(tri_src), _p(tri_layer), int(cap_tri), _p(counts), _p(seconds),
+    )
+    c = dict(zip(CPU_COUNT_FIELDS, map(int, counts)))
+    P, T = min(c["polygons"], cap_poly), min(c["triangles"], cap_tri)
+    rings = [poly_xy[i, : poly_n[i]].copy() for i in range(P)]
+    return CpuResult(
+        rings, poly_src[:P].copy(), poly_layer[:P].copy(), tri_uv[:T].copy(), tri_world[:T].copy(),
+        tri_src[:T].copy(), tri_layer[:T].copy(), c, seconds.copy(),
+    )
+
+
+def cpu_baseline(host, seconds=20.0, threads=1, workload=""):
+    """Time full CPU peels of ``host`` for about ``seconds`` (at least one run);
+    report ms per full peel (median)."""
+    import os
+    import time
+
+    times, runs, start = [], 0, time.perf_counter()
+    result = None
+    while runs == 0 or (time.perf_counter() - start < seconds and runs < 50):
+        t = time.perf_counter()
+        result = cpu_event_peel(*host[:4], threads=threads)
+        times.append(time.perf_counter() - t)
+        runs += 1
+    times.sort()
+    return {
+        "value": round(1e3 * times[len(times) // 2], 3),
+        "unit": "ms",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": f"{runs} full peels of {workload} ({len(host[1])} faces), median; "
+        f"host nproc={os.cpu_count()}",
+        "stage_seconds": [round(x, 4) for x in result.seconds],
+        "polygons": result.counts["polygons"],
+        "layers": result.counts["layers"],
+    }
