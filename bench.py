@@ -40,6 +40,12 @@ def parse():
     ap.add_argument("--extras", action="store_true", help="also time nested shells and the stack")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument(
+        "--copy-calibration",
+        type=int,
+        metavar="MIB",
+        help="only run a plain device copy of MIB mebibytes (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)",
+    )
     return ap.parse_args()
 
 
@@ -145,6 +151,15 @@ def main():
     import torch
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 8 * n, "launches": 3}))
+        return
     peeler, host, info, stages, timing, total, nverts = run_config(args.config, args.steps, args.warmup)
     hot = {k: v for k, v in stages.items() if k.startswith("k_")}
     dominant = max(hot, key=hot.get)

@@ -1,6 +1,7 @@
 // C-ABI: workspace lifetime, eager warm-up, graph capture and replay.
 // Declarations and reference citations: include/peel355.h.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -198,11 +199,19 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   ALLOC(w.hids, H);
   ALLOC(w.hids_alt, H);
   long BV = w.cap.vertices, T = w.cap.triangles;
+  ALLOC(w.owners, BV);
+  ALLOC(w.starts, BV);
+  ALLOC(w.ends, BV);
+  ALLOC(w.partners, BV);
+  ALLOC(w.parents, R);
+  w.ehash2_size = pow2_at_least(2 * (int)std::min<long>(BV, 1L << 29));
+  ALLOC(w.eslots, w.ehash2_size);
   ALLOC(w.vcounts, R);
   ALLOC(w.voffsets, R);
   ALLOC(w.bverts, BV);
   ALLOC(w.blayers, BV);
   ALLOC(w.tasks, BV);
+  ALLOC(w.crowded, BV);
   ALLOC(w.point_boxes, BV);
   ALLOC(w.pflag, BV);
   ALLOC(w.poff, BV);
@@ -227,7 +236,13 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device));
   int per_cu = subdivide_blocks_per_cu();
-  w.persistent_blocks = std::max(1, cus * std::max(1, per_cu));
+  // One 256-lane block per CU: enough lanes to hide traversal latency, few
+  // enough idle pollers that the queue words do not become an L2 hot spot.
+  w.persistent_blocks = std::max(1, cus * std::max(1, std::min(per_cu, 1)));
+  if (const char *env = std::getenv("PEEL355_PERSISTENT_BLOCKS")) {
+    int v = std::atoi(env);
+    if (v > 0) w.persistent_blocks = std::min(v, cus * std::max(1, per_cu));
+  }
   hipError_t e = hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     peel_destroy(ws);

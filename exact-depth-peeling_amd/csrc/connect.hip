@@ -1,17 +1,244 @@
-// Stage 3a: rank propagation across certified patch adjacencies
-// (warp_peel/components.py:15-271). Placeholder launcher until the
-// connectivity kernels land: every patch is its own representative.
+// Stage 3a: rank propagation across certified patch adjacencies.
+//
+// Reference semantics (warp_peel/components.py, alecjacobson/exact-depth-peeling):
+//  connection condition  components.py:32-52 (splits > 0 or an internal edge)
+//  patch edge owners     components.py:80-120 (topological owner key)
+//  hash matching         components.py:123-159 (<= 64 probes, misses counted)
+//  certified connect     components.py:162-243 (midpoint contour query,
+//                        lock-free min-index union-find)
+//  roots / identity      components.py:246-271
+//
+// Patch edges are tasks (polygon, edge) at the scanned ring offsets. An edge's
+// owner is the welded key of the internal mesh edge it lies on, or 2^63 |
+// source for a source-local cut, so only exact reversed twins on the same
+// topological edge can match. A match is accepted only if no contour at the
+// shared edge's midpoint is nearer than or level with the patch (padded), and
+// then the two patches join one component whose representative casts the
+// single ray for all of them.
+#include <algorithm>
+
 #include "workspace.hpp"
 
 namespace peel {
 
-__global__ void k_identity(Ws w) {
+__device__ __forceinline__ int npolys_c(const Ws &w) {
+  int p = load_i(w.status + ST_POLYS);
+  return p < w.cap.regions ? p : w.cap.regions;
+}
+
+__device__ __forceinline__ bool connecting(const Ws &w) {
+  return w.status[ST_ERR] == 0 && load_i(w.control + C_CONNECT) != 0;
+}
+
+__global__ void k_connect_reset(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < w.cap.regions) w.reps[i] = i;
+  for (int j = i; j < w.ehash2_size; j += gridDim.x * blockDim.x) w.eslots[j] = -1;
+  for (int j = i; j < w.cap.regions; j += gridDim.x * blockDim.x) {
+    w.parents[j] = j;
+    w.reps[j] = j;
+  }
+  for (int j = i; j < w.cap.vertices; j += gridDim.x * blockDim.x) w.partners[j] = -1;
+  if (i == 0) {
+    // components.py:40-52: connect only after a split or with an internal edge
+    int any = (w.metrics[M_SPLITS] > 0) || (w.control[C_INTERNAL] != 0);
+    w.control[C_CONNECT] = any;
+    if (!any) w.metrics[M_COMPONENTS] = npolys_c(w);
+  }
+}
+
+__global__ void k_patch_counts(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.cap.regions) return;
+  w.vcounts[i] = (i < npolys_c(w) && w.status[ST_ERR] == 0) ? w.out_n[i] : 0;
+}
+
+__global__ void k_patch_edges(Ws w) {
+  int total = w.control[C_PEDGES];
+  if (total > w.cap.vertices) set_error(w.status, ERR_VERTS);
+}
+
+// edge_owners (components.py:80-120)
+__global__ void k_edge_owners(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npolys_c(w) || !connecting(w)) return;
+  int n = w.out_n[i];
+  int base = w.voffsets[i];
+  if (base + n > w.cap.vertices) return;
+  int source = w.out_src[i];
+  int qn = w.src_n[source];
+  long R = w.cap.regions;
+  for (int k = 0; k < n; ++k) {
+    double2 a = w.out_xy[k * R + i], b = w.out_xy[((k + 1) % n) * R + i];
+    unsigned long long owner = 0x8000000000000000ull | (unsigned long long)(uint32_t)source;
+    if (w.clipped[source] == 0) {
+      for (int j = 0; j < qn; ++j) {
+        int edge = w.facing[source] < 0 ? (4 - j) % 3 : j;
+        if (w.masks[3 * source + edge] == 0) {
+          double2 u = w.src_xy[(long)j * w.nf + source], v = w.src_xy[(long)((j + 1) % qn) * w.nf + source];
+          // identity checks also avoid cancellation at shared endpoints
+          if ((same(a, u) || same(a, v) || cross3(u, v, a) == 0.0) &&
+              (same(b, u) || same(b, v) || cross3(u, v, b) == 0.0)) {
+            int e = 3 * source + edge;
+            int f = e / 3, jj = e % 3;
+            int ca = w.canonical[w.faces[3 * f + jj]];
+            int cb = w.canonical[w.faces[3 * f + (jj + 1) % 3]];
+            owner = ((unsigned long long)(uint32_t)min(ca, cb) << 32) | (uint32_t)max(ca, cb);
+            break;
+          }
+        }
+      }
+    }
+    int slot = base + k;
+    w.owners[slot] = owner;
+    w.starts[slot] = a;
+    w.ends[slot] = b;
+    w.tasks[slot] = make_int2(i, k);
+  }
+}
+
+__device__ __forceinline__ uint64_t point_key(double2 x) {
+  // quantisation only places the key in the table; full coordinates decide
+  return mix64((uint64_t)(long long)(x.x * 1.0e12)) ^ mix64((uint64_t)(long long)(x.y * 1.0e12) + 19ull);
+}
+
+// match_edges (components.py:123-159)
+__global__ __launch_bounds__(256) void k_match_edges(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = i < w.control[C_PEDGES] && i < w.cap.vertices && connecting(w);
+  bool matched = false, missed = false;
+  if (active) {
+    double2 a = w.starts[i], b = w.ends[i];
+    if (!same(a, b)) {
+      unsigned long long owner = w.owners[i];
+      uint64_t key = mix64(owner) ^ point_key(a) ^ point_key(b);
+      int mask = w.ehash2_size - 1;
+      int slot = (int)(key & (uint64_t)mask);
+      missed = true;
+      for (int attempt = 0; attempt < 64; ++attempt) {
+        int other = atomicCAS(&w.eslots[slot], -1, i);
+        if (other < 0) {
+          missed = false;
+          break;
+        }
+        __threadfence();
+        if (w.owners[other] == owner && w.tasks[i].x != w.tasks[other].x && same(a, w.ends[other]) &&
+            same(b, w.starts[other])) {
+          w.partners[i] = other;
+          w.partners[other] = i;
+          matched = true;
+          missed = false;
+          break;
+        }
+        slot = (slot + 1) & mask;
+      }
+    }
+  }
+  wave_add_metric(&w.metrics[M_MATCHED], matched);
+  wave_add_metric(&w.metrics[M_HASH_MISSES], missed);
+}
+
+__device__ int find_root(int *parents, int i) {
+  int node = i;
+  while (true) {
+    int parent = load_i(parents + node);
+    if (parent == node) break;
+    int grand = load_i(parents + parent);
+    atomicMin(parents + node, grand);
+    node = grand;
+  }
+  return node;
+}
+
+__device__ void unite(int *parents, int a, int b) {
+  int ra = find_root(parents, a), rb = find_root(parents, b);
+  while (ra != rb) {
+    int high = max(ra, rb), low = min(ra, rb);
+    int old = atomicMin(parents + high, low);
+    if (old == high) break;
+    ra = find_root(parents, old);
+    rb = find_root(parents, low);
+  }
+}
+
+// connect (components.py:162-243)
+__global__ __launch_bounds__(256) void k_connect(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = i < w.control[C_PEDGES] && i < w.cap.vertices && connecting(w);
+  int other = active ? w.partners[i] : -1;
+  active = active && other > i;
+  long long visits = 0;
+  bool linked = false;
+  if (active) {
+    int2 task = w.tasks[i];
+    double2 a = w.starts[i], b = w.ends[i];
+    if (dlength(make_double2(b.x - a.x, b.y - a.y)) > 16.0 * EPS) {
+      double2 x = make_double2((a.x + b.x) * 0.5, (a.y + b.y) * 0.5);
+      double z = value(w.scores[w.out_src[task.x]], x);
+      double pad = 2.0e-7 * fmax(1.0, fabs(z));
+      float4 lo = make_float4((float)x.x - 2.0e-7f, (float)x.y - 2.0e-7f, -1.0e30f, 0.f);
+      float4 hi = make_float4((float)x.x + 2.0e-7f, (float)x.y + 2.0e-7f, (float)(z + pad), 0.f);
+      Tree tree = w.contour_tree.tree();
+      int node = 1;
+      bool safe = true;
+      while (node > 0) {
+        int c = query_next(tree, lo, hi, node);
+        if (c < 0) break;
+        visits += 1;
+        double2 u = w.cseg_a[c], v = w.cseg_b[c];
+        double2 d = make_double2(v.x - u.x, v.y - u.y);
+        double length = dlength(d);
+        if (length > 0.0) {
+          double t = ((x.x - u.x) * d.x + (x.y - u.y) * d.y) / (d.x * d.x + d.y * d.y);
+          if (fabs(cross3(u, v, x)) <= EPS * length && t >= -EPS / length && t <= 1.0 + EPS / length) {
+            double n0 = w.cnear[c], n1 = w.cnear[w.nseg + c];
+            double depth = n0 + t * (n1 - n0);
+            if (depth <= z + pad) {
+              safe = false;
+              break;
+            }
+          }
+        }
+      }
+      if (safe) {
+        unite(w.parents, task.x, w.tasks[other].x);
+        linked = true;
+      }
+    }
+  }
+  wave_add_metric(&w.metrics[M_CONNECTION_TESTS], visits);
+  wave_add_metric(&w.metrics[M_CONNECTIONS], linked);
+}
+
+// roots (components.py:246-258)
+__global__ __launch_bounds__(256) void k_roots(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = i < npolys_c(w) && connecting(w);
+  bool root = false;
+  if (active) {
+    int r = find_root(w.parents, i);
+    w.reps[i] = r;
+    root = r == i;
+  }
+  wave_add_metric(&w.metrics[M_COMPONENTS], root);
+}
+
+static inline dim3 grid_for(long n, int block = 256) {
+  long g = (n + block - 1) / block;
+  return dim3((unsigned)(g < 1 ? 1 : g));
 }
 
 void enqueue_connect(const Ws &w, hipStream_t s) {
-  hipLaunchKernelGGL(k_identity, dim3((w.cap.regions + 255) / 256), dim3(256), 0, s, w);
+  long reset_n = std::max<long>(std::max<long>(w.ehash2_size, w.cap.regions), w.cap.vertices);
+  long g = (reset_n + 255) / 256;
+  hipLaunchKernelGGL(k_connect_reset, dim3((unsigned)(g > 4096 ? 4096 : g)), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_patch_counts, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  scan_exclusive(s, w.vcounts, w.voffsets, nullptr, w.cap.regions, w.cap.regions,
+                 w.control + C_PEDGES, w.scan);
+  hipLaunchKernelGGL(k_patch_edges, dim3(1), dim3(1), 0, s, w);
+  hipLaunchKernelGGL(k_edge_owners, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_match_edges, grid_for(w.cap.vertices), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_connect, grid_for(w.cap.vertices), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_roots, grid_for(w.cap.regions), dim3(256), 0, s, w);
 }
 
 }  // namespace peel

@@ -23,6 +23,7 @@ __device__ __forceinline__ int npolys_m(const Ws &w) {
 
 __global__ void k_vertex_counts(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) w.control[C_CROWDED] = 0;
   if (i >= w.cap.regions) return;
   int c = 0;
   if (i < npolys_m(w) && w.status[ST_ERR] == 0) c = w.out_n[i];
@@ -136,8 +137,9 @@ __device__ __forceinline__ void lift(const Ws &w, double2 x, double4 sc, double 
 
 constexpr int EDGE_MAX = 128;
 
-// Heapsort in place (kernels.py:1066-1088).
-__device__ void heapsort(double *params, int count) {
+// Heapsort in place (kernels.py:1066-1088); Array is any indexable store.
+template <class Array>
+__device__ void heapsort(Array &params, int count) {
   int start = count / 2 - 1, end = count - 1;
   while (end > 0) {
     int root = 0;
@@ -165,114 +167,280 @@ __device__ void heapsort(double *params, int count) {
   }
 }
 
-// One lane per polygon edge (kernels.py:1011-1124). Phase 1 gathers and sorts
-// the edge parameters and counts the fan triangles; one atomic per wave then
-// reserves the wave's triangle slots; phase 2 writes uv and lifted corners.
+// Parameters of the same-layer vertices strictly inside edge (a, b):
+// t in (EPS, 1 - EPS) and distance <= EPS * |b - a| (kernels.py:1040-1064).
+struct EdgeQuery {
+  double2 a, b, d;
+  double length2, len;
+  float4 lo, hi;
+  __device__ bool accept(double2 x, double &t) const {
+    t = ((x.x - a.x) * d.x + (x.y - a.y) * d.y) / length2;
+    return t > EPS && t < 1.0 - EPS && fabs(cross3(a, b, x)) <= EPS * len;
+  }
+};
+
+constexpr int REG_PARAMS = 8;
+
+// Sorting network on 8 registers (Batcher odd-even merge), static indices only.
+__device__ __forceinline__ void cswap(double &x, double &y) {
+  double lo = fmin(x, y), hi = fmax(x, y);
+  x = lo;
+  y = hi;
+}
+__device__ __forceinline__ void sort8(double (&p)[REG_PARAMS]) {
+  cswap(p[0], p[1]); cswap(p[2], p[3]); cswap(p[4], p[5]); cswap(p[6], p[7]);
+  cswap(p[0], p[2]); cswap(p[1], p[3]); cswap(p[4], p[6]); cswap(p[5], p[7]);
+  cswap(p[1], p[2]); cswap(p[5], p[6]); cswap(p[0], p[4]); cswap(p[3], p[7]);
+  cswap(p[1], p[5]); cswap(p[2], p[6]);
+  cswap(p[1], p[4]); cswap(p[3], p[6]);
+  cswap(p[2], p[4]); cswap(p[3], p[5]);
+  cswap(p[3], p[4]);
+}
+
+// Per-lane parameter array in LDS, strided by the block width so the lanes of
+// a wave touch consecutive banks.
+constexpr int CROWD_THREADS = 64;
+struct LdsParams {
+  double *base;  // &lds[0][lane]
+  __device__ double &operator[](int k) { return base[k * CROWD_THREADS]; }
+  __device__ double operator[](int k) const { return base[k * CROWD_THREADS]; }
+};
+
+// Crowded edges (more than REG_PARAMS interior vertices): gather up to
+// edge_vertices parameters and heapsort them, as the reference does for every
+// edge.
+__device__ int gather_crowded(const Ws &w, const EdgeQuery &q, LdsParams &params) {
+  Tree tree = w.point_tree.tree();
+  int node = 1, count = 0;
+  while (node > 0) {
+    int k = query_next(tree, q.lo, q.hi, node);
+    if (k < 0) break;
+    double t;
+    if (q.accept(w.upts[k], t)) {
+      if (count < w.cap.edge_vertices) params[count] = t;
+      count += 1;
+    }
+  }
+  if (count <= w.cap.edge_vertices) heapsort(params, count);
+  return count;
+}
+
+// Fan walk over the sorted parameters: counts (emit == false) or writes
+// the triangles starting at `slot`.
+template <class Param>
+__device__ __forceinline__ int fan(const Ws &w, const EdgeQuery &q, double2 center, int count,
+                                   Param param, bool emit, int slot, int source, int layer,
+                                   double4 sc) {
+  double previous = 0.0;
+  double2 cursor = q.a;
+  int ntri = 0;
+  for (int step = 0; step <= count; ++step) {
+    double following = step < count ? param(step) : 1.0;
+    if (following <= previous + EPS) continue;
+    double2 next = make_double2(q.a.x + following * q.d.x, q.a.y + following * q.d.y);
+    if (fabs(cross3(center, cursor, next)) > EPS * EPS) {
+      if (emit) {
+        int k = slot + ntri;
+        if (k < w.cap.triangles) {
+          double2 tri[3] = {center, cursor, next};
+          double *uv = w.tri_uv + 6L * k;
+          double *world = w.tri_world + 9L * k;
+          for (int c = 0; c < 3; ++c) {
+            uv[2 * c] = tri[c].x;
+            uv[2 * c + 1] = tri[c].y;
+            lift(w, tri[c], sc, world + 3 * c);
+          }
+          w.tri_src[k] = source;
+          w.tri_layer[k] = layer;
+        } else {
+          set_error(w.status, ERR_TRIS);
+        }
+      }
+      ntri += 1;
+    }
+    cursor = next;
+    previous = following;
+  }
+  return ntri;
+}
+
+__device__ __forceinline__ bool edge_setup(const Ws &w, int j, EdgeQuery &q, double2 &center, int &i,
+                                           int &layer) {
+  int2 task = w.tasks[j];
+  i = task.x;
+  int edge = task.y;
+  int n = w.out_n[i];
+  if (edge >= n) return false;
+  long R = w.cap.regions;
+  q.a = w.out_xy[edge * R + i];
+  q.b = w.out_xy[((edge + 1) % n) * R + i];
+  q.d = make_double2(q.b.x - q.a.x, q.b.y - q.a.y);
+  q.length2 = q.d.x * q.d.x + q.d.y * q.d.y;
+  q.len = sqrt(q.length2);
+  center = make_double2(0.0, 0.0);
+  for (int k = 0; k < n; ++k) {
+    double2 v = w.out_xy[k * R + i];
+    center.x += v.x / (double)n;
+    center.y += v.y / (double)n;
+  }
+  layer = w.out_layer[i];
+  q.lo = make_float4((float)fmin(q.a.x, q.b.x) - 2.0e-7f, (float)fmin(q.a.y, q.b.y) - 2.0e-7f, (float)layer, 0.f);
+  q.hi = make_float4((float)fmax(q.a.x, q.b.x) + 2.0e-7f, (float)fmax(q.a.y, q.b.y) + 2.0e-7f, (float)layer, 0.f);
+  return true;
+}
+
+// One lane per polygon edge (kernels.py:1011-1124). Up to REG_PARAMS edge
+// parameters stay in registers (static indices, sorting network); an edge with
+// more is queued for k_mesh_crowded. Phase 1 counts the fan triangles, one
+// atomic per wave reserves the wave's slots, phase 2 writes uv and lifted
+// world corners.
 __global__ __launch_bounds__(256) void k_mesh_edges(Ws w) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid = j < load_i(w.status + ST_BVERTS) && w.status[ST_ERR] == 0;
-  int i = 0, edge = 0, n = 0;
+  EdgeQuery q;
+  double2 center = make_double2(0, 0);
+  int i = 0, layer = 0, count = 0;
+  bool crowded = false;
+  double p[REG_PARAMS];
+#pragma unroll
+  for (int k = 0; k < REG_PARAMS; ++k) p[k] = 2.0;
+  if (valid) valid = edge_setup(w, j, q, center, i, layer);
   if (valid) {
-    int2 task = w.tasks[j];
-    i = task.x;
-    edge = task.y;
-    n = w.out_n[i];
-    valid = edge < n;
-  }
-  long R = w.cap.regions;
-  double2 a = make_double2(0, 0), b = a, d = a, center = a;
-  double length2 = 0.0;
-  int layer = 0;
-  double params[EDGE_MAX];
-  int count = 0;
-  if (valid) {
-    a = w.out_xy[edge * R + i];
-    b = w.out_xy[((edge + 1) % n) * R + i];
-    d = make_double2(b.x - a.x, b.y - a.y);
-    length2 = d.x * d.x + d.y * d.y;
-    for (int k = 0; k < n; ++k) {
-      double2 p = w.out_xy[k * R + i];
-      center.x += p.x / (double)n;
-      center.y += p.y / (double)n;
-    }
-    layer = w.out_layer[i];
-    float4 lo = make_float4((float)fmin(a.x, b.x) - 2.0e-7f, (float)fmin(a.y, b.y) - 2.0e-7f, (float)layer, 0.f);
-    float4 hi = make_float4((float)fmax(a.x, b.x) + 2.0e-7f, (float)fmax(a.y, b.y) + 2.0e-7f, (float)layer, 0.f);
     Tree tree = w.point_tree.tree();
-    int node = 1;
-    double len = sqrt(length2);
-    while (node > 0 && length2 > 0.0) {
-      int k = query_next(tree, lo, hi, node);
+    int node = q.length2 > 0.0 ? 1 : 0;
+    while (node > 0) {
+      int k = query_next(tree, q.lo, q.hi, node);
       if (k < 0) break;
-      double2 x = w.upts[k];
-      double t = ((x.x - a.x) * d.x + (x.y - a.y) * d.y) / length2;
-      if (t > EPS && t < 1.0 - EPS && fabs(cross3(a, b, x)) <= EPS * len) {
-        if (count < w.cap.edge_vertices) {
-          params[count++] = t;
-        } else {
-          count += 1;
+      double t;
+      if (q.accept(w.upts[k], t)) {
+#pragma unroll
+        for (int r = 0; r < REG_PARAMS; ++r) p[r] = (r == count) ? t : p[r];
+        count += 1;
+        if (count > REG_PARAMS) {
+          crowded = true;
           break;
         }
       }
     }
-    if (count > w.cap.edge_vertices) {
-      set_error(w.status, ERR_EDGE);
-      valid = false;
-    }
+    sort8(p);
+  }
+  int slot = wave_claim32(w.control + C_CROWDED, crowded);
+  if (crowded) {
+    if (slot < w.cap.vertices) w.crowded[slot] = j;
+    valid = false;
   }
   int top = wave_max(valid ? count : 0);
   if ((threadIdx.x & 63) == 0 && top > 0)
     atomicMax((unsigned long long *)&w.metrics[M_EDGE_MAX], (unsigned long long)top);
-  if (valid) heapsort(params, count);
-  // phase 1: count emitted triangles
-  int ntri = 0;
-  if (valid) {
-    double previous = 0.0;
-    double2 cursor = a;
-    for (int step = 0; step <= count; ++step) {
-      double following = step < count ? params[step] : 1.0;
-      if (following <= previous + EPS) continue;
-      double2 next = make_double2(a.x + following * d.x, a.y + following * d.y);
-      if (fabs(cross3(center, cursor, next)) > EPS * EPS) ntri += 1;
-      cursor = next;
-      previous = following;
-    }
-  }
+  auto reg = [&](int k) {
+    double v = p[0];
+#pragma unroll
+    for (int r = 1; r < REG_PARAMS; ++r) v = (k == r) ? p[r] : v;
+    return v;
+  };
+  int source = valid ? w.out_src[i] : 0;
+  double4 sc = valid ? w.scores[source] : make_double4(0, 0, 0, 0);
+  int ntri = valid ? fan(w, q, center, count, reg, false, 0, 0, 0, sc) : 0;
   int total;
   int offset = wave_prefix(ntri, total);
   int base = 0;
   if ((threadIdx.x & 63) == 0 && total > 0) base = atomicAdd(w.status + ST_TRIS, total);
   base = __shfl(base, 0, 64);
   if (!valid || ntri == 0) return;
-  // phase 2: emit
-  int source = w.out_src[i];
-  double4 sc = w.scores[source];
-  double previous = 0.0;
-  double2 cursor = a;
-  int slot = base + offset;
-  for (int step = 0; step <= count; ++step) {
-    double following = step < count ? params[step] : 1.0;
-    if (following <= previous + EPS) continue;
-    double2 next = make_double2(a.x + following * d.x, a.y + following * d.y);
-    if (fabs(cross3(center, cursor, next)) > EPS * EPS) {
-      if (slot < w.cap.triangles) {
-        double2 tri[3] = {center, cursor, next};
-        double *uv = w.tri_uv + 6L * slot;
-        double *world = w.tri_world + 9L * slot;
-        for (int c = 0; c < 3; ++c) {
-          uv[2 * c] = tri[c].x;
-          uv[2 * c + 1] = tri[c].y;
-          lift(w, tri[c], sc, world + 3 * c);
+  fan(w, q, center, count, reg, true, base + offset, source, layer, sc);
+}
+
+// Crowded edges, one wave each. Lane l owns the parameter range
+// [l/64, (l+1)/64) of the edge and queries only the box of that sub-segment
+// (padded like the whole-edge box, so every vertex the whole-edge query would
+// accept is found by exactly one lane). The lanes' sorted runs concatenate into
+// the sorted parameter list in LDS; lane 0 then walks the fan. Up to
+// edge_vertices parameters; more sets error bit 128 (kernels.py:1051-1057).
+constexpr int LANE_PARAMS = 16;
+
+__global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
+  __shared__ double lds[4][EDGE_MAX];
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double *list = lds[wave];
+  int n = load_i(w.control + C_CROWDED);
+  if (n > w.cap.vertices) n = w.cap.vertices;
+  int waves = gridDim.x * 4;
+  for (int t = blockIdx.x * 4 + wave; t < n; t += waves) {
+    int j = w.crowded[t];
+    EdgeQuery q;
+    double2 center;
+    int i, layer;
+    if (!edge_setup(w, j, q, center, i, layer)) continue;  // wave-uniform
+    double t0 = lane / 64.0, t1 = (lane + 1) / 64.0;
+    double2 p0 = make_double2(q.a.x + t0 * q.d.x, q.a.y + t0 * q.d.y);
+    double2 p1 = make_double2(q.a.x + t1 * q.d.x, q.a.y + t1 * q.d.y);
+    float4 lo = make_float4((float)fmin(p0.x, p1.x) - 2.0e-7f, (float)fmin(p0.y, p1.y) - 2.0e-7f, q.lo.z, 0.f);
+    float4 hi = make_float4((float)fmax(p0.x, p1.x) + 2.0e-7f, (float)fmax(p0.y, p1.y) + 2.0e-7f, q.hi.z, 0.f);
+    double mine[LANE_PARAMS];
+    int count = 0;
+    Tree tree = w.point_tree.tree();
+    int node = q.length2 > 0.0 ? 1 : 0;
+    while (node > 0) {
+      int k = query_next(tree, lo, hi, node);
+      if (k < 0) break;
+      double tt;
+      if (q.accept(w.upts[k], tt) && tt >= t0 && (tt < t1 || lane == 63)) {
+        // insertion into the lane's sorted run (static indices only)
+        if (count < LANE_PARAMS) {
+#pragma unroll
+          for (int r = LANE_PARAMS - 1; r >= 0; --r) {
+            bool prev_gt = r > 0 && r - 1 < count && mine[r > 0 ? r - 1 : 0] > tt;
+            double prev = mine[r > 0 ? r - 1 : 0];
+            if (r < count)
+              mine[r] = prev_gt ? prev : (mine[r] > tt ? tt : mine[r]);
+            else if (r == count)
+              mine[r] = prev_gt ? prev : tt;
+          }
         }
-        w.tri_src[slot] = source;
-        w.tri_layer[slot] = layer;
-      } else {
-        set_error(w.status, ERR_TRIS);
+        count += 1;
       }
-      slot += 1;
     }
-    cursor = next;
-    previous = following;
+    int total;
+    int offset = wave_prefix(count, total);
+    if (lane == 0) atomicMax((unsigned long long *)&w.metrics[M_EDGE_MAX], (unsigned long long)total);
+    if (total > w.cap.edge_vertices) {
+      if (lane == 0) set_error(w.status, ERR_EDGE);
+      continue;
+    }
+#pragma unroll
+    for (int r = 0; r < LANE_PARAMS; ++r)
+      if (r < count && count <= LANE_PARAMS) list[offset + r] = mine[r];
+    if (count > LANE_PARAMS) {
+      // a lane with a dense sub-range gathers again straight into LDS and
+      // insertion-sorts its run there
+      int node2 = 1, m = 0;
+      while (node2 > 0) {
+        int k = query_next(tree, lo, hi, node2);
+        if (k < 0) break;
+        double tt;
+        if (q.accept(w.upts[k], tt) && tt >= t0 && (tt < t1 || lane == 63)) list[offset + m++] = tt;
+      }
+      for (int a = 1; a < m; ++a) {
+        double v = list[offset + a];
+        int b = a - 1;
+        while (b >= 0 && list[offset + b] > v) {
+          list[offset + b + 1] = list[offset + b];
+          --b;
+        }
+        list[offset + b + 1] = v;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+      auto mem = [&](int k) { return list[k]; };
+      int source = w.out_src[i];
+      double4 sc = w.scores[source];
+      int ntri = fan(w, q, center, total, mem, false, 0, 0, 0, sc);
+      int base = atomicAdd(w.status + ST_TRIS, ntri);
+      fan(w, q, center, total, mem, true, base, source, layer, sc);
+    }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -309,6 +477,8 @@ void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t) {
   mark(t, s, "point_hierarchy");
   hipLaunchKernelGGL(k_mesh_edges, grid_for(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "k_mesh_edges");
+  hipLaunchKernelGGL(k_mesh_crowded, dim3(1024), dim3(256), 0, s, w);
+  mark(t, s, "k_mesh_crowded");
 }
 
 }  // namespace peel

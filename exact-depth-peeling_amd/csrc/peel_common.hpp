@@ -91,6 +91,10 @@ enum Control : int {
   C_HUGE = 132,     // any group too large for a workgroup sort
   C_NSEG = 133,     // unique contour segments
   C_NUP = 134,      // unique boundary vertices
+  C_INTERNAL = 135, // any internal (non-contour) mesh edge
+  C_CONNECT = 136,  // connectivity pass enabled for this run
+  C_PEDGES = 137,   // patch edges (scan total)
+  C_CROWDED = 138,  // edges with more noding candidates than the register path
   C_COUNT = 256,
 };
 

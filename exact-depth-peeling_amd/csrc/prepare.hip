@@ -384,6 +384,7 @@ __global__ void k_edge_mark(Ws w) {
     slot = (slot + 1) & mask;
   }
   w.masks[e] = contour;
+  if (!contour) w.control[C_INTERNAL] = 1;
 }
 
 __device__ __forceinline__ bool contour_edge(const Ws &w, int source, int edge) {

@@ -160,7 +160,7 @@ __device__ __forceinline__ void ray_box(double2 x, float4 &lo, float4 &hi) {
 // their unvisited depth-first suffix to the subtree-parallel kernel. The
 // reference counts a group as heavy once it passes 32 candidates
 // (event_kernels.py:222-245); that counter keeps its meaning.
-constexpr int HANDOFF = 256;
+constexpr int HANDOFF = 32;
 
 // Ray leaders in Morton order of their samples, so the lanes of a wave walk
 // the same hierarchy nodes (the reference sorts its queries by coordinates,
@@ -214,8 +214,12 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   bool ok = w.status[ST_ERR] == 0;
   Tree tree = w.source_tree.tree();
   int jobs = ok ? load_i(w.control + C_HEAVY) : 0;
+  // Subtree partition of each job's unvisited suffix (event_kernels.py:258-262),
+  // sized so jobs * parts stays near the worker count; with many jobs each one
+  // simply continues from its cursor (parts = 1) instead of re-entering 16
+  // subtrees from their roots.
   int parts = min(8192, tree.leaves);
-  while (parts > 16 && (long long)parts * jobs > 65536) parts /= 2;
+  while (parts > 1 && (long long)parts * jobs > 65536) parts /= 2;
   int width = tree.leaves / parts;
   long total = (long)jobs * parts;
   long long visits = 0;

@@ -79,13 +79,21 @@ __device__ int clean_slot(double2 *xy, int stride, int n) {
   return count;
 }
 
+// A lane's current ring lives in LDS, vertex-major so the 64 lanes of a wave
+// read vertex j of their rings from consecutive banks.
+constexpr int SUB_THREADS = 256;
+struct LdsRing {
+  double2 *base;  // &ring[0][lane]
+  __device__ double2 operator[](int j) const { return base[j * SUB_THREADS]; }
+};
+
 // frontier.py:81-103
-__device__ bool front_segment(const Slot &s, int n, double2 a, double2 b, double za, double zb) {
+__device__ bool front_segment(const LdsRing &s, int n, double2 a, double2 b, double za, double zb) {
   double low = 0.0, high = 1.0;
   for (int j = 0; j <= n; ++j) {
     double da = za, db = zb;
     if (j < n) {
-      double2 u = vtx(s, j), v = vtx(s, (j + 1) % n);
+      double2 u = s[j], v = s[(j + 1) % n];
       da = cross3(u, v, a);
       db = cross3(u, v, b);
     }
@@ -103,9 +111,9 @@ __device__ __forceinline__ int append_slot(double2 *xy, int stride, int n, doubl
 
 // save_polygon (kernels.py:579-596) with layer 0; ranks are assigned later.
 // The output cursor is claimed once per wave.
-__device__ void save_output(const Ws &w, const Slot &s, int n, int source, int o) {
+__device__ void save_output(const Ws &w, const LdsRing &s, int n, int source, int o) {
   if (o < w.cap.regions) {
-    for (int j = 0; j < n; ++j) w.out_xy[(long)j * w.cap.regions + o] = vtx(s, j);
+    for (int j = 0; j < n; ++j) w.out_xy[(long)j * w.cap.regions + o] = s[j];
     w.out_n[o] = n;
     w.out_src[o] = source;
     w.out_layer[o] = 0;
@@ -114,7 +122,9 @@ __device__ void save_output(const Ws &w, const Slot &s, int n, int source, int o
   }
 }
 
-__global__ __launch_bounds__(256) void k_subdivide(Ws w) {
+__global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
+  __shared__ double2 ring_lds[MAXV * SUB_THREADS];
+  const LdsRing ring{ring_lds + threadIdx.x};
   const int nf = w.nf;
   const Tree tree = w.contour_tree.tree();
   int lane = threadIdx.x & 63;
@@ -160,6 +170,9 @@ __global__ __launch_bounds__(256) void k_subdivide(Ws w) {
         if (n == 0) {
           atomicSub(w.control + C_PENDING, 1);
           task = -1;
+        } else {
+          Slot src = slot_of(w, task);
+          for (int j = 0; j < n && j < MAXV; ++j) ring.base[j * SUB_THREADS] = vtx(src, j);
         }
       }
     }
@@ -182,11 +195,11 @@ __global__ __launch_bounds__(256) void k_subdivide(Ws w) {
       set_error(w.status, ERR_ROUNDS);
       break;
     }
-    Slot s = slot_of(w, task);
+    const LdsRing &s = ring;
     double4 score = w.scores[source];
     double lx = 1e30, ly = 1e30, lz = 1e30, hx = -1e30, hy = -1e30, hz = -1e30;
     for (int j = 0; j < n; ++j) {
-      double2 x = vtx(s, j);
+      double2 x = s[j];
       double z = value(score, x);
       lx = fmin(lx, x.x), ly = fmin(ly, x.y), lz = fmin(lz, z);
       hx = fmax(hx, x.x), hy = fmax(hy, x.y), hz = fmax(hz, z);
@@ -205,7 +218,7 @@ __global__ __launch_bounds__(256) void k_subdivide(Ws w) {
       double2 a = w.cseg_a[c], b = w.cseg_b[c];
       bool positive = false, negative = false;
       for (int j = 0; j < n; ++j) {
-        double side = cross3(a, b, vtx(s, j));
+        double side = cross3(a, b, s[j]);
         positive |= side > 0.0;
         negative |= side < 0.0;
       }
@@ -225,7 +238,7 @@ __global__ __launch_bounds__(256) void k_subdivide(Ws w) {
       int left = 0, right = 0;
       double2 d = make_double2(b.x - a.x, b.y - a.y);
       for (int j = 0; j < n; ++j) {
-        double2 u = vtx(s, j), v = vtx(s, (j + 1) % n);
+        double2 u = s[j], v = s[(j + 1) % n];
         double du = -d.y * (u.x - a.x) + d.x * (u.y - a.y) + 0.0;
         double dv = -d.y * (v.x - a.x) + d.x * (v.y - a.y) + 0.0;
         if (du >= 0.0) left = append_slot(L, fcap, left, u);
@@ -248,7 +261,7 @@ __global__ __launch_bounds__(256) void k_subdivide(Ws w) {
         splits += 1;
       } else {
         // A collapsed child cancels the cut: carry the parent (frontier.py:200-206).
-        for (int j = 0; j < n; ++j) L[(long)j * fcap] = vtx(s, j);
+        for (int j = 0; j < n; ++j) L[(long)j * fcap] = s[j];
         left = n;
         right = 0;
       }
@@ -301,7 +314,7 @@ void enqueue_subdivide(const Ws &w, hipStream_t s, Timeline *t) {
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_frontier_reset, dim3((unsigned)grid), dim3(256), 0, s, w);
   mark(t, s, "frontier_reset");
-  hipLaunchKernelGGL(k_subdivide, dim3(w.persistent_blocks), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_subdivide, dim3(w.persistent_blocks), dim3(SUB_THREADS), 0, s, w);
   mark(t, s, "k_subdivide");
   hipLaunchKernelGGL(k_subdivide_finish, dim3(1), dim3(1), 0, s, w);
 }
@@ -311,7 +324,7 @@ void enqueue_subdivide(const Ws &w, hipStream_t s, Timeline *t) {
 namespace peel {
 int subdivide_blocks_per_cu() {
   int blocks = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_subdivide, 256, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_subdivide, SUB_THREADS, 0) != hipSuccess)
     return 2;
   return blocks;
 }

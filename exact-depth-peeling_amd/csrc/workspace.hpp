@@ -94,11 +94,20 @@ struct Ws {
   uint64_t *hkeys, *hkeys_alt;  // [hits]
   int *hids, *hids_alt;         // [hits]
 
+  // connectivity (patch edges at the scanned ring offsets, [vertices])
+  unsigned long long *owners;
+  double2 *starts, *ends;
+  int *partners;
+  int *parents;            // [regions] union-find
+  int *eslots;             // [ehash2_size] reversed-edge hash
+  int ehash2_size;
+
   // meshing
   int *vcounts, *voffsets;  // [regions]
   double2 *bverts;          // [vertices]
   int *blayers;             // [vertices]
   int2 *tasks;              // [vertices]
+  int *crowded;             // [vertices] edges for the per-lane-array noding path
   Box *point_boxes;         // [vertices] boxes of the compacted unique vertices
   int *pflag, *poff;        // [vertices] first-copy flags and their scan
   double2 *upts;            // [vertices] unique (layer, point) boundary vertices

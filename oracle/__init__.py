@@ -168,6 +168,8 @@ CPU_COUNT_FIELDS = (
     "contour_tests",
     "unique_contours",
     "boundary_vertices",
+    "rank_components",
+    "rank_connections",
 )
 
 
@@ -180,7 +182,8 @@ class CpuResult:
         self.counts, self.seconds = counts, seconds
 
 
-def cpu_event_peel(vertices, faces, camera, settings, layers=0, threads=0, cap_poly=None, cap_tri=None):
+def cpu_event_peel(vertices, faces, camera, settings, layers=0, threads=0, cap_poly=None, cap_tri=None,
+                   propagate_ranks=True):
     vertices = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
     faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
     camera = np.ascontiguousarray(camera, dtype=np.float64).reshape(4, 4)
@@ -196,8 +199,9 @@ def cpu_event_peel(vertices, faces, camera, settings, layers=0, threads=0, cap_p
     tri_world = np.zeros((cap_tri, 3, 3))
     tri_src = np.zeros(cap_tri, np.int32)
     tri_layer = np.zeros(cap_tri, np.int32)
-    counts = np.zeros(10, np.int64)
+    counts = np.zeros(12, np.int64)
     seconds = np.zeros(4)
+    lib().oracle_cpu_event_options(int(bool(propagate_ranks)))
     f = lib().oracle_cpu_event_peel
     f.restype = ctypes.c_int
     f(

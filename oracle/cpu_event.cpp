@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <array>
 #include <tuple>
@@ -204,6 +205,8 @@ struct Out {
   std::vector<int> src, layer;
 };
 
+bool g_propagate = true;
+
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -212,8 +215,11 @@ double now() {
 
 extern "C" {
 
-// counts[0..9]: polygons, triangles, layers, splits, ray_groups, ray_hits,
-// error bits, contour_tests, unique_contours, boundary vertices.
+void oracle_cpu_event_options(int propagate_ranks) { g_propagate = propagate_ranks != 0; }
+
+// counts[0..11]: polygons, triangles, layers, splits, ray_groups, ray_hits,
+// error bits, contour_tests, unique_contours, boundary vertices, rank
+// components, rank connections.
 // seconds[0..3]: prepare, subdivide, rank, mesh.
 int oracle_cpu_event_peel(const double *V, int nv, const int *F, int nf, const double *M,
                           const double *S, int layers_limit, int threads, double *poly_xy,
@@ -223,7 +229,7 @@ int oracle_cpu_event_peel(const double *V, int nv, const int *F, int nf, const d
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #endif
-  for (int k = 0; k < 10; ++k) counts[k] = 0;
+  for (int k = 0; k < 12; ++k) counts[k] = 0;
   int error = 0;
   double t0 = now();
   // ---- validate (kernels.py:309-350) ------------------------------------
@@ -499,10 +505,112 @@ int oracle_cpu_event_peel(const double *V, int nv, const int *F, int nf, const d
   counts[3] = splits;
   counts[7] = tests;
   double t2 = now();
+  // ---- rank propagation (components.py:15-271) -----------------------------
+  std::vector<int> reps(P);
+  for (int i = 0; i < P; ++i) reps[i] = i;
+  bool any_internal = false;
+  for (int e = 0; e < 3 * nf; ++e) any_internal |= masks[e] == 0;
+  long components = g_propagate ? P : 0, connections = 0;  // identity: components.py:261-271
+  if (g_propagate && (splits > 0 || any_internal)) {
+    struct PE {
+      uint64_t owner;
+      V2 a, b;
+      int poly;
+    };
+    std::vector<PE> edges;
+    for (int i = 0; i < P; ++i) {
+      const Ring &p = out.polys[i];
+      int s = out.src[i];
+      const Ring &q = src[s];
+      for (int k = 0; k < p.n; ++k) {
+        V2 a = p.v[k], b = p.v[(k + 1) % p.n];
+        uint64_t owner = 0x8000000000000000ull | (uint64_t)(uint32_t)s;
+        if (!clipped[s]) {
+          for (int j = 0; j < q.n; ++j) {
+            int edge = facing[s] < 0 ? (4 - j) % 3 : j;
+            if (masks[3 * s + edge] == 0) {
+              V2 u = q.v[j], v = q.v[(j + 1) % q.n];
+              if ((same(a, u) || same(a, v) || cross(u, v, a) == 0.0) &&
+                  (same(b, u) || same(b, v) || cross(u, v, b) == 0.0)) {
+                int ca = canonical[F[3 * s + edge]], cb = canonical[F[3 * s + (edge + 1) % 3]];
+                owner = ((uint64_t)(uint32_t)std::min(ca, cb) << 32) | (uint32_t)std::max(ca, cb);
+                break;
+              }
+            }
+          }
+        }
+        edges.push_back({owner, a, b, i});
+      }
+    }
+    // exact reversed twins on the same owner (components.py:123-159)
+    auto key = [](uint64_t o, V2 a, V2 b) {
+      return std::make_tuple(o, a.x == 0.0 ? 0.0 : a.x, a.y == 0.0 ? 0.0 : a.y, b.x == 0.0 ? 0.0 : b.x,
+                             b.y == 0.0 ? 0.0 : b.y);
+    };
+    std::map<std::tuple<uint64_t, double, double, double, double>, int> open;
+    std::vector<std::pair<int, int>> pairs;
+    for (int e = 0; e < (int)edges.size(); ++e) {
+      const PE &x = edges[e];
+      if (same(x.a, x.b)) continue;
+      auto it = open.find(key(x.owner, x.b, x.a));
+      if (it != open.end() && edges[it->second].poly != x.poly) {
+        pairs.push_back({it->second, e});
+        open.erase(it);
+      } else {
+        open.emplace(key(x.owner, x.a, x.b), e);
+      }
+    }
+    std::vector<int> parent(P);
+    for (int i = 0; i < P; ++i) parent[i] = i;
+    std::function<int(int)> find = [&](int i) {
+      while (parent[i] != i) i = parent[i] = parent[parent[i]];
+      return i;
+    };
+    for (auto &pr : pairs) {
+      const PE &x = edges[pr.first];
+      if (len({x.b.x - x.a.x, x.b.y - x.a.y}) <= 16.0 * EPS) continue;
+      V2 m{(x.a.x + x.b.x) * 0.5, (x.a.y + x.b.y) * 0.5};
+      double z = value(&score[4 * out.src[x.poly]], m);
+      double pad = 2.0e-7 * std::fmax(1.0, std::fabs(z));
+      float lo[3] = {(float)m.x - 2.0e-7f, (float)m.y - 2.0e-7f, -1.0e30f};
+      float hi[3] = {(float)m.x + 2.0e-7f, (float)m.y + 2.0e-7f, (float)(z + pad)};
+      int node = 1;
+      bool safe = true;
+      while (node > 0) {
+        int leaf = contour_tree.next_leaf(lo, hi, node);
+        if (leaf < 0) break;
+        int c = contour_tree.ids[leaf];
+        V2 u = seg_a[c], v = seg_b[c];
+        V2 d{v.x - u.x, v.y - u.y};
+        double length = len(d);
+        if (length > 0.0) {
+          double t = ((m.x - u.x) * d.x + (m.y - u.y) * d.y) / (d.x * d.x + d.y * d.y);
+          if (std::fabs(cross(u, v, m)) <= EPS * length && t >= -EPS / length && t <= 1.0 + EPS / length) {
+            double depth = near0[c] + t * (near1[c] - near0[c]);
+            if (depth <= z + pad) {
+              safe = false;
+              break;
+            }
+          }
+        }
+      }
+      if (!safe) continue;
+      connections += 1;
+      int ra = find(x.poly), rb = find(edges[pr.second].poly);
+      if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);
+    }
+    components = 0;
+    for (int i = 0; i < P; ++i) {
+      reps[i] = find(i);
+      components += reps[i] == i;
+    }
+  }
+  counts[10] = components;
+  counts[11] = connections;
   // ---- ranks (event_kernels.py:28-385) ------------------------------------
   std::vector<V2> samples(P);
   for (int i = 0; i < P; ++i) {
-    const Ring &p = out.polys[i];
+    const Ring &p = out.polys[reps[i]];
     int anchor = 0;
     for (int j = 1; j < p.n; ++j)
       if (lexless(p.v[j], p.v[anchor])) anchor = j;
@@ -561,7 +669,7 @@ int oracle_cpu_event_peel(const double *V, int nv, const int *F, int nf, const d
   counts[5] = ray_hits;
   int layers = 0, limit = layers_limit > 0 ? layers_limit : (1 << 30);
   for (int i = 0; i < P; ++i) {
-    int s = out.src[i];
+    int s = out.src[reps[i]];
     V2 x = samples[i];
     double z = value(&score[4 * s], x);
     const auto &h = hits[group[i]];

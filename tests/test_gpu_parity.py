@@ -49,19 +49,21 @@ def gpu_outputs(p):
     )
 
 
-def compare(host, threads=8):
+def compare(host, threads=8, propagate=True):
     import torch  # noqa: F401
 
     from hip_peel import EventPeeler
     from hip_peel.io import upload
 
-    p = EventPeeler(*upload(*host[:4]))
+    p = EventPeeler(*upload(*host[:4]), propagate_ranks=propagate)
     p.run()
     info, rings, srcs, lays, uv, world, tsrc, tlay = gpu_outputs(p)
-    cpu = oracle.cpu_event_peel(*host[:4], threads=threads)
+    cpu = oracle.cpu_event_peel(*host[:4], threads=threads, propagate_ranks=propagate)
     assert cpu.counts["error"] == 0
-    for key in ("polygons", "triangles", "layers", "splits", "ray_groups", "ray_hits", "unique_contours",
-                "contour_tests"):
+    # ray counters depend on which patch represents a component; the patch
+    # and triangle sets do not
+    for key in ("polygons", "triangles", "layers", "splits", "unique_contours", "contour_tests",
+                "rank_components", "rank_connections"):
         assert info[key] == cpu.counts[key], (key, info[key], cpu.counts[key])
     assert canonical_patches(rings, srcs, lays) == canonical_patches(cpu.rings, cpu.sources, cpu.layers)
     assert canonical_triangles(uv, world, tsrc, tlay) == canonical_triangles(
@@ -70,10 +72,14 @@ def compare(host, threads=8):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("propagate", [True, False])
 @pytest.mark.parametrize("name", CASES)
-def test_fixture_bit_parity_with_cpu_route(name):
-    print(f"[parity] {name}", flush=True)
-    compare(oracle.load_scene(fixture_path(name))[:4])
+def test_fixture_bit_parity_with_cpu_route(name, propagate):
+    print(f"[parity] {name} propagate={propagate}", flush=True)
+    p, info = compare(oracle.load_scene(fixture_path(name))[:4], propagate=propagate)
+    if not propagate:
+        cpu = oracle.cpu_event_peel(*oracle.load_scene(fixture_path(name))[:4], propagate_ranks=False)
+        assert (info["ray_groups"], info["ray_hits"]) == (cpu.counts["ray_groups"], cpu.counts["ray_hits"])
 
 
 @pytest.mark.gpu
