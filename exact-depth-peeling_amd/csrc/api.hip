@@ -107,6 +107,8 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   if (d->layers < 0 || d->layers > d->layer_capacity)
     return fail(6, "layers must be zero (all), or fit in the layer capacity");
   if (d->route != PEEL_ROUTE_EVENTS && d->route != PEEL_ROUTE_LAYERED) return fail(7, "unknown route");
+  if (d->route == PEEL_ROUTE_LAYERED && (d->workers < 1 || d->pieces < 1))
+    return fail(4, "Capacities must be positive");
   PeelWorkspace *ws = new PeelWorkspace();
   HIP_TRY(hipGetDevice(&ws->device));
   Ws &w = ws->w;
@@ -123,6 +125,10 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   w.all_layers = d->layers == 0;
   w.share_regions = (d->flags & PEEL_SHARE_REGIONS) != 0;
   w.propagate_ranks = (d->flags & PEEL_PROPAGATE_RANKS) != 0;
+  w.route = d->route;
+  const bool layered = d->route == PEEL_ROUTE_LAYERED;
+  w.workers = layered ? d->workers : 1;
+  w.pieces = layered ? d->pieces : 1;
   int nf1 = w.nf > 0 ? w.nf : 1, nv1 = w.nv > 0 ? w.nv : 1;
   ALLOC(w.status, 16);
   ALLOC(w.metrics, M_COUNT);
@@ -165,7 +171,25 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   ALLOC(w.leaf_xy, 8L * w.source_tree.size);
   ALLOC(w.leaf_n, w.source_tree.size);
   ALLOC(w.leaf_score, w.source_tree.size);
-  long F = w.cap.frontier, R = w.cap.regions;
+  long F = layered ? 1 : w.cap.frontier, R = w.cap.regions;
+  // layered route: ping-pong residual pools sized by the frontier capacity,
+  // per-worker piece pools, per-layer statistics
+  long RES = layered ? w.cap.frontier : 1, NF = layered ? nf1 : 1;
+  long POOL = layered ? 3L * (w.pieces + 3) * w.workers : 1;
+  ALLOC(w.ranks, NF);
+  ALLOC(w.contour_boxes, NF);
+  for (int par = 0; par < 2; ++par) {
+    ALLOC(w.res_xy[par], RES * MAXV);
+    ALLOC(w.res_n[par], RES);
+    ALLOC(w.res_src[par], RES);
+    ALLOC(w.res_run[par], RES);
+    ALLOC(w.res_link[par], RES);
+    ALLOC(w.res_box[par], RES);
+    ALLOC(w.heads[par], NF);
+  }
+  ALLOC(w.pool_xy, POOL * MAXV);
+  ALLOC(w.pool_n, POOL);
+  ALLOC(w.stats, 3L * w.cap.layers);
   ALLOC(w.fr_xy, F * MAXV);
   ALLOC(w.fr_n, F);
   ALLOC(w.fr_src, F);
@@ -190,6 +214,10 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   ALLOC(w.ray_ids_alt, R);
   ALLOC(w.medium, R);
   ALLOC(w.heavy, R);
+  ALLOC(w.beam_box, 4L * BEAM_CELLS);
+  ALLOC(w.beam_leader, BEAM_CELLS);
+  ALLOC(w.beam_valid, BEAM_CELLS);
+  ALLOC(w.beam_jobs, BEAM_CELLS);
   long H = w.cap.hits;
   ALLOC(w.hit_depth, H);
   ALLOC(w.hit_src, H);
@@ -257,13 +285,20 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   return 0;
 }
 
+static void enqueue_all(const Ws &w, hipStream_t s, Timeline *t) {
+  enqueue_prepare(w, s, t);
+  if (w.route == PEEL_ROUTE_LAYERED) {
+    enqueue_layered(w, s, t);
+  } else {
+    enqueue_subdivide(w, s, t);
+    enqueue_rank(w, s, t);
+  }
+  enqueue_mesh(w, s, t);
+}
+
 int peel_enqueue(PeelWorkspace *ws) {
   if (!ws) return fail(1, "null workspace");
-  const Ws &w = ws->w;
-  enqueue_prepare(w, ws->stream);
-  enqueue_subdivide(w, ws->stream);
-  enqueue_rank(w, ws->stream);
-  enqueue_mesh(w, ws->stream);
+  enqueue_all(ws->w, ws->stream, nullptr);
   HIP_TRY(hipGetLastError());
   return 0;
 }
@@ -274,10 +309,7 @@ int peel_profile(PeelWorkspace *ws, int32_t *count, float *ms, char *names, int3
   for (int i = 0; i < Timeline::MAX; ++i) HIP_TRY(hipEventCreate(&t.events[i]));
   const Ws &w = ws->w;
   mark(&t, ws->stream, "begin");
-  enqueue_prepare(w, ws->stream, &t);
-  enqueue_subdivide(w, ws->stream, &t);
-  enqueue_rank(w, ws->stream, &t);
-  enqueue_mesh(w, ws->stream, &t);
+  enqueue_all(w, ws->stream, &t);
   hipError_t e = hipStreamSynchronize(ws->stream);
   std::string joined;
   int n = 0;
@@ -383,6 +415,8 @@ int peel_get_buffers(PeelWorkspace *ws, PeelBuffers *b) {
   b->metrics_count = M_COUNT;
   b->canonical = w.canonical;
   b->stream = (void *)ws->stream;
+  b->stats = (int64_t *)w.stats;
+  b->stats_rows = w.cap.layers;
   return 0;
 }
 

@@ -30,14 +30,19 @@ __device__ __forceinline__ bool connecting(const Ws &w) {
   return w.status[ST_ERR] == 0 && load_i(w.control + C_CONNECT) != 0;
 }
 
+__device__ __forceinline__ int pedges(const Ws &w) {
+  int total = load_i(w.control + C_PEDGES);
+  return total < w.cap.vertices ? total : w.cap.vertices;
+}
+__device__ __forceinline__ int eslots_live(const Ws &w) { return live_table(pedges(w), w.ehash2_size); }
+
 __global__ void k_connect_reset(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int j = i; j < w.ehash2_size; j += gridDim.x * blockDim.x) w.eslots[j] = -1;
-  for (int j = i; j < w.cap.regions; j += gridDim.x * blockDim.x) {
+  int P = npolys_c(w);
+  for (int j = i; j < P; j += gridDim.x * blockDim.x) {
     w.parents[j] = j;
     w.reps[j] = j;
   }
-  for (int j = i; j < w.cap.vertices; j += gridDim.x * blockDim.x) w.partners[j] = -1;
   if (i == 0) {
     // components.py:40-52: connect only after a split or with an internal edge
     int any = (w.metrics[M_SPLITS] > 0) || (w.control[C_INTERNAL] != 0);
@@ -55,6 +60,15 @@ __global__ void k_patch_counts(Ws w) {
 __global__ void k_patch_edges(Ws w) {
   int total = w.control[C_PEDGES];
   if (total > w.cap.vertices) set_error(w.status, ERR_VERTS);
+}
+
+// Clear the live part of the reversed-edge table and the partner slots.
+__global__ void k_eslots_reset(Ws w) {
+  if (!connecting(w)) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int size = eslots_live(w), n = pedges(w);
+  for (int j = i; j < size; j += gridDim.x * blockDim.x) w.eslots[j] = -1;
+  for (int j = i; j < n; j += gridDim.x * blockDim.x) w.partners[j] = -1;
 }
 
 // edge_owners (components.py:80-120)
@@ -111,7 +125,7 @@ __global__ __launch_bounds__(256) void k_match_edges(Ws w) {
     if (!same(a, b)) {
       unsigned long long owner = w.owners[i];
       uint64_t key = mix64(owner) ^ point_key(a) ^ point_key(b);
-      int mask = w.ehash2_size - 1;
+      int mask = eslots_live(w) - 1;
       int slot = (int)(key & (uint64_t)mask);
       missed = true;
       for (int attempt = 0; attempt < 64; ++attempt) {
@@ -228,13 +242,14 @@ static inline dim3 grid_for(long n, int block = 256) {
 }
 
 void enqueue_connect(const Ws &w, hipStream_t s) {
-  long reset_n = std::max<long>(std::max<long>(w.ehash2_size, w.cap.regions), w.cap.vertices);
-  long g = (reset_n + 255) / 256;
-  hipLaunchKernelGGL(k_connect_reset, dim3((unsigned)(g > 4096 ? 4096 : g)), dim3(256), 0, s, w);
+  long g = ((long)w.cap.regions + 255) / 256;
+  hipLaunchKernelGGL(k_connect_reset, dim3((unsigned)(g > 1024 ? 1024 : g)), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_patch_counts, grid_for(w.cap.regions), dim3(256), 0, s, w);
   scan_exclusive(s, w.vcounts, w.voffsets, nullptr, w.cap.regions, w.cap.regions,
                  w.control + C_PEDGES, w.scan);
   hipLaunchKernelGGL(k_patch_edges, dim3(1), dim3(1), 0, s, w);
+  long eg = ((long)std::max(w.ehash2_size, w.cap.vertices) + 255) / 256;
+  hipLaunchKernelGGL(k_eslots_reset, dim3((unsigned)(eg > 1024 ? 1024 : eg)), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_edge_owners, grid_for(w.cap.regions), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_match_edges, grid_for(w.cap.vertices), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_connect, grid_for(w.cap.vertices), dim3(256), 0, s, w);

@@ -52,15 +52,24 @@ __global__ void k_scatter(Ws w) {
 // Boundary vertices repeat once per incident polygon; only the first copy of
 // each exact (layer, x, y) enters the point hierarchy. Duplicates would give
 // the same edge parameter, which the fan loop skips anyway, so the output is
-// unchanged and each edge query sees each distinct vertex once.
+// unchanged and each edge query sees each distinct vertex once. The
+// edge_vertices capacity therefore bounds distinct vertices per edge, where
+// the reference bounds copies (kernels.py:1051-1057): long contour cuts on
+// the 65,536-face torus knot carry about 90 distinct vertices, about twice as
+// many copies, which a bound of 128 copies would reject.
 __device__ __forceinline__ uint64_t bvert_hash(double2 x, int layer) {
   return mix64(dbits(x.x) ^ mix64(dbits(x.y) + 0x9e3779b97f4a7c15ull) ^
                mix64((uint64_t)(uint32_t)layer + 0x3c6ef372fe94f82aull));
 }
 
+__device__ __forceinline__ int phash_live(const Ws &w) {
+  return live_table(load_i(w.status + ST_BVERTS), w.phash_size);
+}
+
 __global__ void k_phash_reset(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int j = i; j < w.phash_size; j += gridDim.x * blockDim.x) w.phash[j] = -1;
+  int size = phash_live(w);
+  for (int j = i; j < size; j += gridDim.x * blockDim.x) w.phash[j] = -1;
 }
 
 __global__ void k_point_dedupe(Ws w) {
@@ -68,9 +77,9 @@ __global__ void k_point_dedupe(Ws w) {
   if (i >= load_i(w.status + ST_BVERTS) || w.status[ST_ERR] != 0) return;
   double2 x = w.bverts[i];
   int layer = w.blayers[i];
-  int mask = w.phash_size - 1;
+  int size = phash_live(w), mask = size - 1;
   int slot = (int)(bvert_hash(x, layer) & (uint64_t)mask);
-  for (int probe = 0; probe < w.phash_size; ++probe) {
+  for (int probe = 0; probe < size; ++probe) {
     int cur = load_i(&w.phash[slot]);
     if (cur < 0) {
       cur = atomicCAS(&w.phash[slot], -1, i);
@@ -87,9 +96,9 @@ __global__ void k_point_dedupe(Ws w) {
 __device__ __forceinline__ bool first_copy(const Ws &w, int i) {
   double2 x = w.bverts[i];
   int layer = w.blayers[i];
-  int mask = w.phash_size - 1;
+  int size = phash_live(w), mask = size - 1;
   int slot = (int)(bvert_hash(x, layer) & (uint64_t)mask);
-  for (int probe = 0; probe < w.phash_size; ++probe) {
+  for (int probe = 0; probe < size; ++probe) {
     int cur = w.phash[slot];
     if (cur < 0) return true;
     if (same(w.bverts[cur], x) && w.blayers[cur] == layer) return cur == i;
@@ -330,6 +339,10 @@ __global__ __launch_bounds__(256) void k_mesh_edges(Ws w) {
     valid = false;
   }
   int top = wave_max(valid ? count : 0);
+  if (valid && count > w.cap.edge_vertices) {
+    set_error(w.status, ERR_EDGE);
+    valid = false;
+  }
   if ((threadIdx.x & 63) == 0 && top > 0)
     atomicMax((unsigned long long *)&w.metrics[M_EDGE_MAX], (unsigned long long)top);
   auto reg = [&](int k) {
@@ -463,7 +476,7 @@ void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t) {
   hipLaunchKernelGGL(k_scatter, grid_for(w.cap.regions), dim3(256), 0, s, w);
   {
     long g = ((long)w.phash_size + 255) / 256;
-    hipLaunchKernelGGL(k_phash_reset, dim3((unsigned)(g > 2048 ? 2048 : g)), dim3(256), 0, s, w);
+    hipLaunchKernelGGL(k_phash_reset, dim3((unsigned)(g > 1024 ? 1024 : g)), dim3(256), 0, s, w);
   }
   hipLaunchKernelGGL(k_point_dedupe, grid_for(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "scatter_vertices");

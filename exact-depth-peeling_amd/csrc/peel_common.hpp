@@ -20,6 +20,10 @@ constexpr int MAXV = 32;      // ring capacity (warp_peel/kernels.py:10)
 constexpr int SRCV = 8;       // triangle + near plane + 4 window planes
 constexpr double EPS = 1.0e-10;  // warp_peel/kernels.py:11
 constexpr float PAD = 2.0e-7f;   // box padding (warp_peel/kernels.py:235-250)
+constexpr int ROUTE_EVENTS = 0;  // event route (EventPeeler)
+constexpr int ROUTE_LAYERED = 1;  // per-layer clipping route (DevicePeeler)
+constexpr int BEAM_SIDE = 32;    // window cells per axis for shared rays (warp_peel/beams.py:18-22)
+constexpr int BEAM_CELLS = BEAM_SIDE * BEAM_SIDE;
 
 // status[16] slots and error bits: warp_peel/kernels.py:17-21, events.py:811-814.
 enum Status : int {
@@ -95,6 +99,8 @@ enum Control : int {
   C_CONNECT = 136,  // connectivity pass enabled for this run
   C_PEDGES = 137,   // patch edges (scan total)
   C_CROWDED = 138,  // edges with more noding candidates than the register path
+  C_BEAM_JOBS = 139, // cells whose certification continues in subtree workers
+  C_LOOP = 140,      // layered route: another peeling pass is due
   C_COUNT = 256,
 };
 
@@ -152,6 +158,17 @@ __device__ __forceinline__ double unkey(uint64_t k) {
 // Atomic min on binary64 through the order-preserving key (values are finite).
 __device__ __forceinline__ void atomic_min_key(unsigned long long *p, double x) {
   atomicMin(p, (unsigned long long)dkey(x));
+}
+
+// Live size of an open-addressing table: the power of two at least twice the
+// live key count (load <= 1/2), at least 256, at most the allocated power of
+// two. Tables are cleared and probed only over their live size, so a replay
+// with few keys touches few lines.
+__device__ __forceinline__ int live_table(long count, int capacity) {
+  long need = 2 * count;
+  int size = 256;
+  while (size < capacity && size < need) size <<= 1;
+  return size < capacity ? size : capacity;
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t x) {
@@ -277,6 +294,36 @@ __device__ __forceinline__ int query_leaf(const Tree &t, float4 lo, float4 hi, i
     }
   }
   return -1;
+}
+
+// clean (frontier.py:45-71 = kernels.py:146-176): in-place cleanup of a
+// strided ring (vertex j at xy[j * stride]); returns the new count.
+__device__ inline int clean_slot(double2 *xy, int stride, int n) {
+  if (n > MAXV) return n;
+  int count = 0;
+  double2 last = make_double2(0, 0);
+  for (int j = 0; j < n; ++j) {
+    double2 x = xy[(long)j * stride];
+    if (count == 0 || dlength(make_double2(x.x - last.x, x.y - last.y)) > EPS * 0.01) {
+      xy[(long)count * stride] = x;
+      last = x;
+      count += 1;
+    }
+  }
+  if (count > 1) {
+    double2 first = xy[0];
+    if (dlength(make_double2(first.x - last.x, first.y - last.y)) <= EPS * 0.01) count -= 1;
+  }
+  if (count < 3) return 0;
+  double2 origin = xy[0];
+  double area2 = 0.0, perimeter = 0.0;
+  for (int j = 0; j < count; ++j) {
+    double2 a = xy[(long)j * stride], b = xy[(long)((j + 1) % count) * stride];
+    area2 += cross3(origin, a, b);
+    perimeter += fabs(b.x - a.x) + fabs(b.y - a.y);
+  }
+  if (area2 <= EPS * perimeter) return 0;
+  return count;
 }
 
 // Padded binary32 box of a ring evaluated with its depth plane

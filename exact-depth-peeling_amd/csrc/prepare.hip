@@ -521,6 +521,12 @@ void enqueue_prepare(const Ws &w, hipStream_t s, Timeline *t) {
     hipLaunchKernelGGL(k_edge_mark, grid_for(3L * w.nf), dim3(256), 0, s, w);
   }
   mark(t, s, "weld_project_topology");
+  if (w.route == ROUTE_LAYERED) {
+    Hierarchy st = w.source_tree;
+    build_hierarchy(s, w.src_boxes, nullptr, w.nf, w.nf, false, st, w.sort);
+    mark(t, s, "source_hierarchy");
+    return;
+  }
   hipLaunchKernelGGL(k_segments, grid_for(w.nseg), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_segments_reduce, grid_for(w.nseg), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_segment_flags, grid_for(w.nseg), dim3(256), 0, s, w);

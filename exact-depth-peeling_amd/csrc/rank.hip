@@ -3,6 +3,8 @@
 // Reference semantics (file:line in alecjacobson/exact-depth-peeling):
 //  samples          warp_peel/event_kernels.py:28-48
 //  exact groups     warp_peel/event_kernels.py:51-92 (group_queries)
+//  shared rays      warp_peel/beams.py:18-207 (cell, group, covers, certify,
+//                   certify_heavy, apply)
 //  contains_flat    warp_peel/event_kernels.py:131-148 (half-open ownership)
 //  store_hit        warp_peel/event_kernels.py:151-169
 //  ray_hits         warp_peel/event_kernels.py:196-245
@@ -11,8 +13,9 @@
 //  assign_ranks     warp_peel/event_kernels.py:342-385
 //
 // Samples with identical coordinates share one ray group; the group leader is
-// the smallest patch index holding that sample (hash table + atomicMin instead
-// of the reference's two float64 radix sorts and a binary search).
+// the patch that claimed the sample's hash slot first (a hash table instead of
+// the reference's two float64 radix sorts and a binary search). Which member
+// leads does not matter: all members cast the same ray.
 #include "workspace.hpp"
 
 namespace peel {
@@ -26,10 +29,22 @@ __device__ __forceinline__ uint64_t point_hash(double2 x) {
   return mix64(dbits(x.x) ^ mix64(dbits(x.y) + 0x9e3779b97f4a7c15ull));
 }
 
+__device__ __forceinline__ int qhash_live(const Ws &w) { return live_table(npolys(w), w.qhash_size); }
+
 __global__ void k_rank_reset(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int j = i; j < w.qhash_size; j += gridDim.x * blockDim.x) w.qhash[j] = -1;
+  int size = qhash_live(w);
+  for (int j = i; j < size; j += gridDim.x * blockDim.x) w.qhash[j] = -1;
+  for (int j = i; j < BEAM_CELLS; j += gridDim.x * blockDim.x) {
+    w.beam_box[j] = ~0ull;
+    w.beam_box[BEAM_CELLS + j] = ~0ull;
+    w.beam_box[2 * BEAM_CELLS + j] = 0ull;
+    w.beam_box[3 * BEAM_CELLS + j] = 0ull;
+    w.beam_leader[j] = 0x7fffffff;
+    w.beam_valid[j] = 0;
+  }
   if (i == 0) {
+    w.control[C_BEAM_JOBS] = 0;
     w.control[C_HEAVY] = 0;
     w.control[C_MEDIUM] = 0;
     w.control[C_HUGE] = 0;
@@ -65,21 +80,25 @@ __global__ void k_samples(Ws w) {
   w.group_counts[i] = 0;
   w.group_fill[i] = 0;
   __threadfence();
-  int mask = w.qhash_size - 1;
+  int size = qhash_live(w), mask = size - 1;
   int slot = (int)(point_hash(x) & (uint64_t)mask);
-  for (int probe = 0; probe < w.qhash_size; ++probe) {
+  for (int probe = 0; probe < size; ++probe) {
     int cur = load_i(&w.qhash[slot]);
     if (cur < 0) {
       cur = atomicCAS(&w.qhash[slot], -1, i);
       if (cur < 0) return;
     }
     __threadfence();
-    if (same(w.samples[cur], x)) {
-      if (i < cur) atomicMin(&w.qhash[slot], i);
-      return;
-    }
+    if (same(w.samples[cur], x)) return;
     slot = (slot + 1) & mask;
   }
+}
+
+// cell (beams.py:18-22): 32 x 32 buckets of the unit window, clamped
+__device__ __forceinline__ int beam_cell(double2 x) {
+  double cx = fmin(fmax(x.x * (double)BEAM_SIDE, 0.0), (double)(BEAM_SIDE - 1));
+  double cy = fmin(fmax(x.y * (double)BEAM_SIDE, 0.0), (double)(BEAM_SIDE - 1));
+  return (int)cx * BEAM_SIDE + (int)cy;
 }
 
 __global__ void k_groups(Ws w) {
@@ -87,10 +106,10 @@ __global__ void k_groups(Ws w) {
   int P = npolys(w);
   if (i >= P || w.status[ST_ERR] != 0) return;
   double2 x = w.samples[i];
-  int mask = w.qhash_size - 1;
+  int size = qhash_live(w), mask = size - 1;
   int slot = (int)(point_hash(x) & (uint64_t)mask);
   int leader = i;
-  for (int probe = 0; probe < w.qhash_size; ++probe) {
+  for (int probe = 0; probe < size; ++probe) {
     int cur = w.qhash[slot];
     if (cur < 0) break;
     if (same(w.samples[cur], x)) {
@@ -100,6 +119,134 @@ __global__ void k_groups(Ws w) {
     slot = (slot + 1) & mask;
   }
   w.groups[i] = leader;
+  if (w.share_regions && leader == i) {
+    // beams.py:53-85: the cell's bounding rectangle of samples and its leader
+    // (one contribution per distinct sample)
+    int c = beam_cell(x);
+    atomicMin(&w.beam_box[c], (unsigned long long)dkey(x.x));
+    atomicMin(&w.beam_box[BEAM_CELLS + c], (unsigned long long)dkey(x.y));
+    atomicMax(&w.beam_box[2 * BEAM_CELLS + c], (unsigned long long)dkey(x.x));
+    atomicMax(&w.beam_box[3 * BEAM_CELLS + c], (unsigned long long)dkey(x.y));
+    atomicMin(&w.beam_leader[c], i);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Shared rays over certified rectangles (beams.py). Samples fall into 32 x 32
+// cells of the unit window. A cell whose samples are not all identical
+// proposes one ray for all its patches; the proposal is accepted only if every
+// source whose padded box meets the samples' bounding rectangle strictly
+// contains all four corners (margin EPS * edge length). On resolved input the
+// set of covering sources and their depth order are then constant over the
+// rectangle, so one ray orders them for every patch in the cell.
+__device__ __forceinline__ double2 beam_corner(const Ws &w, int c, int k) {
+  return make_double2(unkey(w.beam_box[(k & 1 ? 2 : 0) * BEAM_CELLS + c]),
+                      unkey(w.beam_box[(k & 2 ? 3 : 1) * BEAM_CELLS + c]));
+}
+
+__device__ __forceinline__ void beam_box(const Ws &w, int c, float4 &lo, float4 &hi) {
+  double2 a = beam_corner(w, c, 0), b = beam_corner(w, c, 3);
+  lo = make_float4((float)a.x - PAD, (float)a.y - PAD, -1.0e30f, 0.f);
+  hi = make_float4((float)b.x + PAD, (float)b.y + PAD, 1.0e30f, 0.f);
+}
+
+// covers (beams.py:88-97) on a leaf-ordered source record
+__device__ bool covers_leaf(const Ws &w, int leaf, const double2 *corner) {
+  int n = w.leaf_n[leaf];
+  const double2 *r = w.leaf_xy + 8L * leaf;
+  for (int j = 0; j < n; ++j) {
+    double2 a = r[j], b = r[(j + 1) % n];
+    double margin = EPS * dlength(make_double2(b.x - a.x, b.y - a.y));
+    for (int k = 0; k < 4; ++k)
+      if (cross3(a, b, corner[k]) <= margin) return false;
+  }
+  return n > 0;
+}
+
+// certify (beams.py:109-147): one lane per cell, the first HANDOFF candidates
+__global__ __launch_bounds__(256) void k_certify(Ws w) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  bool ok = c < BEAM_CELLS && w.status[ST_ERR] == 0 && w.beam_leader[c] != 0x7fffffff;
+  double2 corner[4];
+  if (ok) {
+    for (int k = 0; k < 4; ++k) corner[k] = beam_corner(w, c, k);
+    // identical samples already share their exact group
+    ok = !(corner[0].x == corner[3].x && corner[0].y == corner[3].y);
+  }
+  int node = ok ? 1 : 0, visits = 0;
+  bool accepted = ok;
+  Tree tree = w.source_tree.tree();
+  float4 lo, hi;
+  if (ok) beam_box(w, c, lo, hi);
+  for (int step = 0; step < 32 && node > 0; ++step) {
+    int leaf = query_leaf(tree, lo, hi, node);
+    if (leaf < 0) break;
+    visits += 1;
+    if (!covers_leaf(w, leaf, corner)) {
+      accepted = false;
+      break;
+    }
+  }
+  if (ok) w.beam_valid[c] = accepted;
+  bool handoff = accepted && node > 0;
+  if (handoff)
+    while (node < tree.leaves) node *= 2;
+  int slot = wave_claim32(w.control + C_BEAM_JOBS, handoff);
+  if (handoff) w.beam_jobs[slot] = make_int2(c, node);
+  wave_add_metric(&w.metrics[M_BEAM_ATTEMPTS], ok);
+  wave_add_metric(&w.metrics[M_BEAM_TESTS], visits);
+  wave_add_metric(&w.metrics[M_HEAVY_BEAMS], handoff);
+}
+
+// certify_heavy (beams.py:150-189): the unvisited suffix of each handed-off
+// cell in disjoint leaf subtrees; a failure anywhere rejects the cell.
+__global__ __launch_bounds__(256) void k_certify_heavy(Ws w, int workers) {
+  bool ok = w.status[ST_ERR] == 0;
+  Tree tree = w.source_tree.tree();
+  int jobs = ok ? load_i(w.control + C_BEAM_JOBS) : 0;
+  int parts = min(8192, tree.leaves);
+  while (parts > 16 && (long long)parts * jobs > 65536) parts /= 2;
+  int width = tree.leaves / parts;
+  long total = (long)jobs * parts;
+  long long visits = 0;
+  for (long task = (long)blockIdx.x * blockDim.x + threadIdx.x; task < total; task += workers) {
+    int2 job = w.beam_jobs[task / parts];
+    int c = job.x, part = (int)(task % parts);
+    int first = tree.leaves + part * width;
+    if (job.y >= first + width || load_i(w.beam_valid + c) == 0) continue;
+    int root = parts + part;
+    int node = job.y > first ? job.y : root;
+    int end = successor(root);
+    double2 corner[4];
+    for (int k = 0; k < 4; ++k) corner[k] = beam_corner(w, c, k);
+    float4 lo, hi;
+    beam_box(w, c, lo, hi);
+    while (node > 0 && node != end) {
+      int leaf = query_leaf(tree, lo, hi, node, end);
+      if (leaf < 0) break;
+      visits += 1;
+      if (!covers_leaf(w, leaf, corner)) {
+        atomicExch(w.beam_valid + c, 0);
+        break;
+      }
+    }
+  }
+  wave_add_metric(&w.metrics[M_BEAM_TESTS], visits);
+}
+
+// apply (beams.py:192-207): every patch of a certified cell joins its leader
+__global__ __launch_bounds__(256) void k_beam_apply(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool certified = false;
+  if (i < npolys(w) && w.status[ST_ERR] == 0) {
+    int c = beam_cell(w.samples[i]);
+    if (w.beam_valid[c]) {
+      int leader = w.beam_leader[c];
+      w.groups[i] = leader;
+      certified = leader == i;
+    }
+  }
+  wave_add_metric(&w.metrics[M_BEAMS], certified);
 }
 
 // Source rings in hierarchy leaf order: leaf k holds its ring as one 128-byte
@@ -444,13 +591,18 @@ void enqueue_connect(const Ws &w, hipStream_t s);
 
 void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   long qgrid = ((long)w.qhash_size + 255) / 256;
-  if (qgrid > 2048) qgrid = 2048;
+  if (qgrid > 1024) qgrid = 1024;
   hipLaunchKernelGGL(k_rank_reset, dim3((unsigned)qgrid), dim3(256), 0, s, w);
   if (w.propagate_ranks) enqueue_connect(w, s);
   mark(t, s, "connectivity");
   hipLaunchKernelGGL(k_leaf_sources, grid_for(w.source_tree.size), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_samples, grid_for(w.cap.regions), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_groups, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  if (w.share_regions) {
+    hipLaunchKernelGGL(k_certify, dim3(BEAM_CELLS / 256), dim3(256), 0, s, w);
+    hipLaunchKernelGGL(k_certify_heavy, dim3(32), dim3(256), 0, s, w, 32 * 256);
+    hipLaunchKernelGGL(k_beam_apply, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  }
   hipLaunchKernelGGL(k_ray_keys, grid_for(w.cap.regions), dim3(256), 0, s, w);
   radix_sort(s, w.ray_keys, w.ray_ids, w.ray_keys_alt, w.ray_ids_alt, w.status + ST_POLYS, 0,
              w.cap.regions, 0, 32, w.sort);

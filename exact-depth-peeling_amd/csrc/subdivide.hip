@@ -50,34 +50,6 @@ __device__ __forceinline__ Slot slot_of(const Ws &w, int task) {
 
 __device__ __forceinline__ double2 vtx(const Slot &s, int j) { return s.xy[(long)j * s.stride]; }
 
-// frontier.py:45-71: in-place cleanup of a frontier slot, returns the new count.
-__device__ int clean_slot(double2 *xy, int stride, int n) {
-  if (n > MAXV) return n;
-  int count = 0;
-  double2 last = make_double2(0, 0);
-  for (int j = 0; j < n; ++j) {
-    double2 x = xy[(long)j * stride];
-    if (count == 0 || dlength(make_double2(x.x - last.x, x.y - last.y)) > EPS * 0.01) {
-      xy[(long)count * stride] = x;
-      last = x;
-      count += 1;
-    }
-  }
-  if (count > 1) {
-    double2 first = xy[0];
-    if (dlength(make_double2(first.x - last.x, first.y - last.y)) <= EPS * 0.01) count -= 1;
-  }
-  if (count < 3) return 0;
-  double2 origin = xy[0];
-  double area2 = 0.0, perimeter = 0.0;
-  for (int j = 0; j < count; ++j) {
-    double2 a = xy[(long)j * stride], b = xy[(long)((j + 1) % count) * stride];
-    area2 += cross3(origin, a, b);
-    perimeter += fabs(b.x - a.x) + fabs(b.y - a.y);
-  }
-  if (area2 <= EPS * perimeter) return 0;
-  return count;
-}
 
 // A lane's current ring lives in LDS, vertex-major so the 64 lanes of a wave
 // read vertex j of their rings from consecutive banks.

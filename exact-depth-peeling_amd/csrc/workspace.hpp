@@ -89,6 +89,11 @@ struct Ws {
   int *ray_ids, *ray_ids_alt;         // [regions] rays in Morton order
   int *medium;             // [regions] groups sorted by one workgroup
   int2 *heavy;             // [regions]
+  // rectangle certification of shared rays over 32 x 32 window cells
+  unsigned long long *beam_box;  // [4][BEAM_CELLS] order-preserving keys: min x, min y, max x, max y
+  int *beam_leader;        // [BEAM_CELLS] smallest patch index in the cell
+  int *beam_valid;         // [BEAM_CELLS] 1 = every covering candidate certified
+  int2 *beam_jobs;         // [BEAM_CELLS] (cell, first unvisited leaf) handed to the subtree workers
   double *hit_depth;       // [hits]
   int *hit_src, *hit_group;  // [hits]
   uint64_t *hkeys, *hkeys_alt;  // [hits]
@@ -117,6 +122,22 @@ struct Ws {
   double *tri_world;        // [triangles][3][3]
   int *tri_src, *tri_layer;
 
+  // layered route (DevicePeeler): whole-face ranks, then one clipping pass
+  // per layer over the residual pieces
+  int route;                // PEEL_ROUTE_EVENTS or PEEL_ROUTE_LAYERED
+  int workers, pieces;      // worker threads and pieces per worker pool
+  int *ranks;               // [nf] whole-face rank, -1 = residual
+  Box *contour_boxes;       // [nf] boxes of each source's contour edges
+  double2 *res_xy[2];       // [MAXV][cap.frontier] ping-pong residual pools
+  int *res_n[2], *res_src[2];
+  int *res_run[2];          // length of the run of pieces starting at a slot
+  int *res_link[2];         // next run of the same source (-1 ends)
+  Box *res_box[2];
+  int *heads[2];            // [nf] first run of each source's residual pieces
+  double2 *pool_xy;         // [3][pieces + 3][MAXV][workers] visible, temporary, hidden
+  int *pool_n;              // [3][pieces + 3][workers]
+  long long *stats;         // [cap.layers][3] targets, candidate tests, fully hidden
+
   SortScratch sort;
   ScanScratch scan;
   int persistent_blocks;    // grid of the persistent kernels
@@ -125,7 +146,7 @@ struct Ws {
 // Optional host-side timeline: hipEvents recorded between stages when the
 // pipeline runs eagerly for a stage profile (never inside a capture).
 struct Timeline {
-  static constexpr int MAX = 48;
+  static constexpr int MAX = 64;
   hipEvent_t events[MAX];
   const char *names[MAX];
   int count = 0;
@@ -141,5 +162,6 @@ void enqueue_prepare(const Ws &w, hipStream_t s, Timeline *t = nullptr);
 void enqueue_subdivide(const Ws &w, hipStream_t s, Timeline *t = nullptr);
 void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t = nullptr);
 void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t = nullptr);
+void enqueue_layered(const Ws &w, hipStream_t s, Timeline *t = nullptr);
 
 }  // namespace peel

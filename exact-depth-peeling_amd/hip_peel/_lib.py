@@ -34,6 +34,8 @@ class PeelDesc(ctypes.Structure):
         ("layers", ctypes.c_int32),
         ("route", ctypes.c_int32),
         ("flags", ctypes.c_int32),
+        ("workers", ctypes.c_int32),
+        ("pieces", ctypes.c_int32),
     ]
 
 
@@ -55,6 +57,8 @@ class PeelBuffers(ctypes.Structure):
         ("metrics_count", ctypes.c_int32),
         ("canonical", ctypes.c_void_p),
         ("stream", ctypes.c_void_p),
+        ("stats", ctypes.c_void_p),
+        ("stats_rows", ctypes.c_int32),
     ]
 
 

@@ -31,9 +31,9 @@ ERROR_LABELS = {
 
 @dataclass(frozen=True)
 class Capacities:
-    """warp_peel/solver.py:10-31. ``residual`` sizes the frontier slots and
-    ``workers``/``pieces_per_worker`` are accepted for interface parity (the
-    work-queue kernels have no per-worker piece pools)."""
+    """warp_peel/solver.py:10-31: residual pieces per layer, output polygons,
+    boundary vertices, triangles, worker threads, pieces per worker pool,
+    layers and edge-noding candidates."""
 
     residual: int
     polygons: int
@@ -113,7 +113,7 @@ class _Workspace:
 
     _route = _lib.ROUTE_EVENTS
 
-    def _open(self, vertices, faces, camera, settings, caps, layers, flags):
+    def _open(self, vertices, faces, camera, settings, caps, layers, flags, workers=1, pieces=1):
         lib = _lib.load()
         if not isinstance(vertices, torch.Tensor) or vertices.device.type != "cuda":
             raise ValueError("Peelers require ROCm device tensors (torch.device('cuda'))")
@@ -143,6 +143,8 @@ class _Workspace:
             layers,
             self._route,
             flags,
+            workers,
+            pieces,
         )
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
@@ -166,6 +168,7 @@ class _Workspace:
         self.status = device_view(b.status, (16,), torch.int32, dev)
         self.metrics = device_view(b.metrics, (b.metrics_count,), torch.int64, dev)
         self.canonical = device_view(b.canonical, (max(1, self.nv),), torch.int32, dev)
+        self.stats = device_view(b.stats, (b.stats_rows, 3), torch.int64, dev)
         self.stream = torch.cuda.ExternalStream(b.stream, device=dev)
         self.graph_captured = False
         self.warmed = False
@@ -267,11 +270,15 @@ class _Workspace:
 
 
 class DevicePeeler(_Workspace):
-    """warp_peel.DevicePeeler interface (solver.py:92-519).
+    """warp_peel.DevicePeeler interface (solver.py:92-519): whole faces with a
+    certain centroid rank are emitted directly, every other face is clipped
+    layer by layer against the sources and residual pieces that may cover it.
 
     Vertices float64 (V, 3), faces int32 (F, 3), camera float64 (1, 4, 4) affine
     world-to-camera, settings float64 (6,) = xmin, ymin, xmax, ymax,
-    perspective (0/1), near. Outputs use dense source indices into faces.
+    perspective (0/1), near. Outputs use dense source indices into faces;
+    ``stats`` holds per layer the targets, candidate tests and fully hidden
+    targets.
     """
 
     _route = _lib.ROUTE_LAYERED
@@ -283,15 +290,14 @@ class DevicePeeler(_Workspace):
         if layers < 0 or layers > cap.layers:
             raise ValueError("layers must be zero (all), or fit in the layer capacity")
         self.cap = cap
-        n = max(16, faces.shape[0] if faces.ndim == 2 else 0)
         caps = (
-            4 * cap.residual,
+            cap.residual,
             cap.polygons,
-            max(32 * n, cap.polygons),
+            1,
             cap.vertices,
             cap.triangles,
-            4096,
+            1,
             cap.layers,
             cap.edge_vertices,
         )
-        self._open(vertices, faces, camera, settings, caps, layers, _lib.SHARE_REGIONS | _lib.PROPAGATE_RANKS)
+        self._open(vertices, faces, camera, settings, caps, layers, 0, cap.workers, cap.pieces_per_worker)

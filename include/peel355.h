@@ -46,7 +46,8 @@ typedef struct PeelDesc {
   int64_t num_faces;
   const double *camera;
   const double *settings;
-  int32_t frontier;       /* frontier slots: seeds + two per contour cut */
+  int32_t frontier;       /* frontier slots: seeds + two per contour cut (event
+                             route); residual pool (layered route) */
   int32_t regions;        /* output polygons */
   int32_t hits;           /* shared ray hits */
   int32_t vertices_cap;   /* boundary vertices */
@@ -57,6 +58,8 @@ typedef struct PeelDesc {
   int32_t layers;         /* 0 = all layers, else prefix length */
   int32_t route;          /* PeelRoute */
   int32_t flags;          /* PeelFlags */
+  int32_t workers;        /* layered route: worker threads (solver.py:10-31 Capacities.workers) */
+  int32_t pieces;         /* layered route: pieces per worker pool (Capacities.pieces_per_worker) */
 } PeelDesc;
 
 /* Device outputs (warp_peel/README.md:186-189 and solver.py:156-200): valid
@@ -80,6 +83,9 @@ typedef struct PeelBuffers {
   int32_t metrics_count;
   int32_t *canonical;  /* [V] welded vertex ids */
   void *stream;        /* hipStream_t owned by the workspace */
+  int64_t *stats;      /* layered route: [stats_rows][3] targets, candidate tests,
+                          fully hidden targets per layer (solver.py:153) */
+  int32_t stats_rows;
 } PeelBuffers;
 
 /* Allocate every pool and create the workspace stream

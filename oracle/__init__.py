@@ -170,6 +170,8 @@ CPU_COUNT_FIELDS = (
     "boundary_vertices",
     "rank_components",
     "rank_connections",
+    "beam_attempts",
+    "certified_beams",
 )
 
 
@@ -183,7 +185,7 @@ class CpuResult:
 
 
 def cpu_event_peel(vertices, faces, camera, settings, layers=0, threads=0, cap_poly=None, cap_tri=None,
-                   propagate_ranks=True):
+                   propagate_ranks=True, share_regions=True):
     vertices = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
     faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
     camera = np.ascontiguousarray(camera, dtype=np.float64).reshape(4, 4)
@@ -199,9 +201,9 @@ def cpu_event_peel(vertices, faces, camera, settings, layers=0, threads=0, cap_p
     tri_world = np.zeros((cap_tri, 3, 3))
     tri_src = np.zeros(cap_tri, np.int32)
     tri_layer = np.zeros(cap_tri, np.int32)
-    counts = np.zeros(12, np.int64)
+    counts = np.zeros(len(CPU_COUNT_FIELDS), np.int64)
     seconds = np.zeros(4)
-    lib().oracle_cpu_event_options(int(bool(propagate_ranks)))
+    lib().oracle_cpu_event_options(int(bool(propagate_ranks)), int(bool(share_regions)))
     f = lib().oracle_cpu_event_peel
     f.restype = ctypes.c_int
     f(

@@ -206,6 +206,7 @@ struct Out {
 };
 
 bool g_propagate = true;
+bool g_share = true;
 
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -215,11 +216,14 @@ double now() {
 
 extern "C" {
 
-void oracle_cpu_event_options(int propagate_ranks) { g_propagate = propagate_ranks != 0; }
+void oracle_cpu_event_options(int propagate_ranks, int share_regions) {
+  g_propagate = propagate_ranks != 0;
+  g_share = share_regions != 0;
+}
 
-// counts[0..11]: polygons, triangles, layers, splits, ray_groups, ray_hits,
+// counts[0..13]: polygons, triangles, layers, splits, ray_groups, ray_hits,
 // error bits, contour_tests, unique_contours, boundary vertices, rank
-// components, rank connections.
+// components, rank connections, beam attempts, certified beams.
 // seconds[0..3]: prepare, subdivide, rank, mesh.
 int oracle_cpu_event_peel(const double *V, int nv, const int *F, int nf, const double *M,
                           const double *S, int layers_limit, int threads, double *poly_xy,
@@ -229,7 +233,7 @@ int oracle_cpu_event_peel(const double *V, int nv, const int *F, int nf, const d
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #endif
-  for (int k = 0; k < 12; ++k) counts[k] = 0;
+  for (int k = 0; k < 14; ++k) counts[k] = 0;
   int error = 0;
   double t0 = now();
   // ---- validate (kernels.py:309-350) ------------------------------------
@@ -630,6 +634,53 @@ int oracle_cpu_event_peel(const double *V, int nv, const int *F, int nf, const d
       auto it = first.emplace(key, i).first;
       group[i] = it->second;
     }
+  }
+  // shared rays over certified cell rectangles (beams.py:18-207): a cell of
+  // the 32 x 32 window grid whose samples are not all identical shares one
+  // ray when every source whose padded box meets the samples' bounding
+  // rectangle strictly contains its four corners
+  if (g_share) {
+    const int side = 32, cells = side * side;
+    std::vector<double> bx0(cells, 1e300), by0(cells, 1e300), bx1(cells, -1e300), by1(cells, -1e300);
+    std::vector<int> leader(cells, -1), cell_of(P);
+    for (int i = 0; i < P; ++i) {
+      V2 x = samples[i];
+      int cx = (int)std::min(std::max(x.x * side, 0.0), (double)(side - 1));
+      int cy = (int)std::min(std::max(x.y * side, 0.0), (double)(side - 1));
+      int c = cx * side + cy;
+      cell_of[i] = c;
+      bx0[c] = std::min(bx0[c], x.x);
+      by0[c] = std::min(by0[c], x.y);
+      bx1[c] = std::max(bx1[c], x.x);
+      by1[c] = std::max(by1[c], x.y);
+      if (leader[c] < 0) leader[c] = i;
+    }
+    std::vector<char> valid(cells, 0);
+    for (int c = 0; c < cells; ++c) {
+      if (leader[c] < 0 || (bx0[c] == bx1[c] && by0[c] == by1[c])) continue;
+      counts[12] += 1;
+      V2 corner[4] = {{bx0[c], by0[c]}, {bx1[c], by0[c]}, {bx0[c], by1[c]}, {bx1[c], by1[c]}};
+      float lo[3] = {(float)bx0[c] - 2.0e-7f, (float)by0[c] - 2.0e-7f, -1.0e30f};
+      float hi[3] = {(float)bx1[c] + 2.0e-7f, (float)by1[c] + 2.0e-7f, 1.0e30f};
+      bool ok = true;
+      int node = 1;
+      while (ok && node > 0) {
+        int leaf = source_tree.next_leaf(lo, hi, node);
+        if (leaf < 0) break;
+        const Ring &r = src[source_tree.ids[leaf]];
+        ok = r.n > 0;
+        for (int j = 0; j < r.n && ok; ++j) {
+          V2 a = r.v[j], b = r.v[(j + 1) % r.n];
+          double margin = 1.0e-10 * std::sqrt((b.x - a.x) * (b.x - a.x) + (b.y - a.y) * (b.y - a.y));
+          for (int k = 0; k < 4 && ok; ++k)
+            if (cross(a, b, corner[k]) <= margin) ok = false;
+        }
+      }
+      valid[c] = ok;
+      counts[13] += ok;
+    }
+    for (int i = 0; i < P; ++i)
+      if (valid[cell_of[i]]) group[i] = leader[cell_of[i]];
   }
   // flat source records in leaf order
   struct Hit {

@@ -52,6 +52,8 @@ def _desc(**kw):
         ({"edge_vertices": 129}, 5, "edge_vertices"),
         ({"layers": 17}, 6, "layers must be zero"),
         ({"route": 9}, 7, "unknown route"),
+        ({"route": 1, "workers": 0, "pieces": 4}, 4, "Capacities must be positive"),
+        ({"route": 1, "workers": 4, "pieces": 0}, 4, "Capacities must be positive"),
         ({"num_faces": -1}, 2, "out of range"),
         ({"camera": None}, 3, "null device array"),
     ],
@@ -76,3 +78,18 @@ def test_null_arguments():
     assert lib.peel_create(None, None) == 1
     assert lib.peel_launch(None) == 1
     assert lib.peel_destroy(None) == 0
+
+
+def _header_fields(struct):
+    text = (ROOT / "include" / "peel355.h").read_text()
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (struct, struct), text, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    return re.findall(r"([A-Za-z_][A-Za-z_0-9]*)\s*;", body)
+
+
+@pytest.mark.parametrize("struct", ["PeelDesc", "PeelBuffers"])
+def test_ctypes_structs_follow_the_header(struct):
+    """The ctypes binding lists the header's fields in the header's order."""
+    from hip_peel import _lib
+
+    assert [f[0] for f in getattr(_lib, struct)._fields_] == _header_fields(struct)

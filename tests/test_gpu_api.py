@@ -264,6 +264,70 @@ def test_nonmanifold_edge():
     check_extraction(p, oracle.load_scene(data)[:4], expected_layers=2)
 
 
+@pytest.mark.parametrize("slanted", [False, True])
+def test_certified_distinct_samples(slanted):
+    """test_events.py:155-190: nested footprints have distinct samples but one
+    certified common order, so one ray serves all of them."""
+    import torch
+
+    from hip_peel import EventCapacities, EventPeeler
+
+    n = 128
+    arrays = stack(n)
+    xyz = arrays[0].cpu().numpy().reshape(n, 3, 3)
+    t = np.arange(n) / n
+    xyz[:, :, :2] *= (1 - 0.1 * t)[:, None, None]
+    xyz[:, :, 0] += 0.01 * t[:, None]
+    if slanted:
+        xyz[:, :, 2] = 10 + 0.001 * np.arange(n)[:, None] + 10 * xyz[:, :, 0]
+    arrays[0] = torch.tensor(xyz.reshape(-1, 3), device="cuda")
+    p = EventPeeler(*arrays)
+    p.run()
+    info = p.read_status()
+    assert info["splits"] == 0
+    assert info["polygons"] == n
+    assert info["certified_beams"] == 1
+    assert info["ray_groups"] == 1
+    assert info["ray_candidate_tests"] == n
+    assert info["beam_candidate_tests"] == n
+    assert info["heavy_beams"] == 1  # 128 candidates pass the 32 handled by one lane
+    np.testing.assert_array_equal(p.output_sources[:n].cpu().numpy(), p.output_layers[:n].cpu().numpy())
+    baseline = EventPeeler(*arrays, share_regions=False, capacities=replace(EventCapacities.for_mesh(n), hits=n * n))
+    baseline.run()
+    control = baseline.read_status()
+    assert control["ray_groups"] == n
+    assert control["ray_candidate_tests"] == n * n
+    assert control["certified_beams"] == 0 and control["beam_attempts"] == 0
+    np.testing.assert_array_equal(
+        baseline.output_sources[:n].cpu().numpy(), baseline.output_layers[:n].cpu().numpy())
+
+
+def test_beam_rejected_when_a_candidate_straddles():
+    """A cell whose rectangle is crossed by a source edge keeps its exact
+    groups (beams.py:88-97: every candidate must strictly cover the corners)."""
+    import torch
+
+    from hip_peel import EventPeeler
+
+    n = 8
+    arrays = stack(n)
+    xyz = arrays[0].cpu().numpy().reshape(n, 3, 3)
+    t = np.arange(n) / n
+    xyz[:, :, :2] *= (1 - 0.1 * t)[:, None, None]
+    xyz[:, :, 0] += 0.01 * t[:, None]
+    # a small far triangle whose edge passes through the cell of the samples
+    extra = np.array([[[0.004, -0.26, 50.0], [0.3, -0.26, 50.0], [0.004, 0.1, 50.0]]])
+    xyz = np.concatenate([xyz, extra])
+    arrays[0] = torch.tensor(xyz.reshape(-1, 3), device="cuda")
+    arrays[1] = torch.tensor(np.arange(3 * (n + 1), dtype=np.int32).reshape(-1, 3), device="cuda")
+    p = EventPeeler(*arrays)
+    p.run()
+    info = p.read_status()
+    assert info["beam_attempts"] >= 1
+    assert info["certified_beams"] == 0
+    check_extraction(p, host_of(arrays))
+
+
 def test_heavy_disjoint_groups():
     """test_events.py:193-206."""
     import torch
@@ -413,3 +477,25 @@ def test_component_switches(name, propagate):
     p.run()
     check_extraction(p, host)
     assert p.read_status()["edge_hash_misses"] == 0
+
+
+def test_event_edge_capacity():
+    """Edge noding capacity (kernels.py:1051-1057, status bit 128): a scene
+    whose busiest edge holds more distinct vertices than edge_vertices fails
+    with the reference's message, on every replay."""
+    from hip_peel import EventCapacities, EventPeeler
+
+    arrays = scene("perspective_torus")[0]
+    p = EventPeeler(*arrays)
+    p.run()
+    busiest = p.read_status()["max_edge_vertices"]
+    assert busiest >= 2
+    cap = replace(EventCapacities.for_mesh(arrays[1].shape[0]), edge_vertices=busiest - 1)
+    q = EventPeeler(*arrays, capacities=cap)
+    for _ in range(2):
+        q.run()
+        with pytest.raises(RuntimeError, match="edge noding capacity"):
+            q.read_status()
+    ok = EventPeeler(*arrays, capacities=replace(cap, edge_vertices=busiest))
+    ok.run()
+    ok.read_status()

@@ -49,21 +49,26 @@ def gpu_outputs(p):
     )
 
 
-def compare(host, threads=8, propagate=True):
+def compare(host, threads=8, propagate=True, share=True):
     import torch  # noqa: F401
 
     from hip_peel import EventPeeler
     from hip_peel.io import upload
 
-    p = EventPeeler(*upload(*host[:4]), propagate_ranks=propagate)
+    p = EventPeeler(*upload(*host[:4]), propagate_ranks=propagate, share_regions=share)
     p.run()
     info, rings, srcs, lays, uv, world, tsrc, tlay = gpu_outputs(p)
-    cpu = oracle.cpu_event_peel(*host[:4], threads=threads, propagate_ranks=propagate)
+    cpu = oracle.cpu_event_peel(*host[:4], threads=threads, propagate_ranks=propagate, share_regions=share)
     assert cpu.counts["error"] == 0
     # ray counters depend on which patch represents a component; the patch
     # and triangle sets do not
-    for key in ("polygons", "triangles", "layers", "splits", "unique_contours", "contour_tests",
-                "rank_components", "rank_connections"):
+    keys = ["polygons", "triangles", "layers", "splits", "unique_contours", "contour_tests",
+            "rank_components", "rank_connections"]
+    if not propagate:
+        # without propagation every patch samples itself, so the ray groups
+        # and certified cells are fixed by the patch set
+        keys += ["ray_groups", "ray_hits", "beam_attempts", "certified_beams"]
+    for key in keys:
         assert info[key] == cpu.counts[key], (key, info[key], cpu.counts[key])
     assert canonical_patches(rings, srcs, lays) == canonical_patches(cpu.rings, cpu.sources, cpu.layers)
     assert canonical_triangles(uv, world, tsrc, tlay) == canonical_triangles(
@@ -72,14 +77,12 @@ def compare(host, threads=8, propagate=True):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("share", [True, False])
 @pytest.mark.parametrize("propagate", [True, False])
 @pytest.mark.parametrize("name", CASES)
-def test_fixture_bit_parity_with_cpu_route(name, propagate):
-    print(f"[parity] {name} propagate={propagate}", flush=True)
-    p, info = compare(oracle.load_scene(fixture_path(name))[:4], propagate=propagate)
-    if not propagate:
-        cpu = oracle.cpu_event_peel(*oracle.load_scene(fixture_path(name))[:4], propagate_ranks=False)
-        assert (info["ray_groups"], info["ray_hits"]) == (cpu.counts["ray_groups"], cpu.counts["ray_hits"])
+def test_fixture_bit_parity_with_cpu_route(name, propagate, share):
+    print(f"[parity] {name} propagate={propagate} share={share}", flush=True)
+    compare(oracle.load_scene(fixture_path(name))[:4], propagate=propagate, share=share)
 
 
 @pytest.mark.gpu
