@@ -8,8 +8,9 @@ projection, topology, hierarchies, subdivision, ranks, noding, fans, lift) on
 synthetic float64 input already resident in HBM; upload, allocation, capture
 and readback are outside the timed region (the reference's protocol,
 scripts/benchmark_events.py:77-93). The headline workload is the 65,536-face
-(2,3) torus-knot tube; --extras adds the 717,740-face nested shells and the
-65,536-layer identical-footprint stack.
+(2,3) torus-knot tube, timed with binary32-filtered ray predicates (`value`)
+and with plain binary64 predicates; the 717,740-face nested shells are timed
+both ways as well. --extras adds the 65,536-layer identical-footprint stack.
 
 Prints one JSON line. ``roofline`` prices the dominant kernel's algorithmic
 bytes per launch against its own event-timed duration; ``cpu_baseline`` times
@@ -37,8 +38,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="torus-knot-65536")
-    ap.add_argument("--extras", action="store_true", help="also time nested shells and the stack")
+    ap.add_argument("--extras", action="store_true", help="also time the identical-footprint stack")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-large", action="store_true", help="skip the 717,740-face nested shells")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument(
         "--copy-calibration",
@@ -49,16 +51,16 @@ def parse():
     return ap.parse_args()
 
 
-def make_peeler(name):
+def make_peeler(name, filter_fp32=False, host=None):
     import torch
 
     from hip_peel import EventPeeler, scenes
     from hip_peel.io import upload
 
-    host = scenes.CONFIGS[name]()
+    host = host if host is not None else scenes.CONFIGS[name]()
     arrays = upload(*host, device="cuda:0")
     torch.cuda.synchronize()
-    return EventPeeler(*arrays), host
+    return EventPeeler(*arrays, filter_fp32=filter_fp32), host
 
 
 def time_replays(peeler, steps, warmup):
@@ -116,14 +118,29 @@ def kernel_bytes(name, host, info, nverts):
     return None
 
 
-def run_config(name, steps, warmup):
-    peeler, host = make_peeler(name)
+def run_config(name, steps, warmup, filter_fp32=False, host=None):
+    peeler, host = make_peeler(name, filter_fp32, host)
     timing = time_replays(peeler, steps, warmup)
     info = peeler.read_status()
     stages = peeler.profile()
     peeler.read_status()
     total, nverts = algorithmic_bytes(peeler, host, info)
     return peeler, host, info, stages, timing, total, nverts
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (scaled by a known-size copy), or None."""
+    path = ROOT / "profiles" / f"traffic_{workload.replace('-', '_')}.md"
+    if not path.exists():
+        return None
+    text = path.read_text()
+    block = text.split("```json", 1)[-1].split("```", 1)[0] if "```json" in text else ""
+    try:
+        table = json.loads(block)["traffic_bytes"]
+    except (ValueError, KeyError):
+        return None
+    return table.get(f"peel::{kernel}")
 
 
 def cpu_baseline(name, seconds):
@@ -160,7 +177,16 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": 8 * n, "launches": 3}))
         return
-    peeler, host, info, stages, timing, total, nverts = run_config(args.config, args.steps, args.warmup)
+    # plain binary64 predicates first, then the binary32-filtered run (`value`)
+    p64, host, info64, stages64, timing64, _, _ = run_config(args.config, args.steps, args.warmup)
+    del p64
+    peeler, host, info, stages, timing, total, nverts = run_config(args.config, args.steps, args.warmup,
+                                                                   filter_fp32=True, host=host)
+    # the patch and triangle sets are deterministic (ray counters depend on
+    # which patch represents a component, so they vary between runs)
+    for key in ("polygons", "triangles", "layers"):
+        assert info[key] == info64[key], (key, info[key], info64[key])
+    decided = info["filtered_predicates"] + info["fp64_fallbacks"]
     hot = {k: v for k, v in stages.items() if k.startswith("k_")}
     dominant = max(hot, key=hot.get)
     kbytes = kernel_bytes(dominant, host, info, nverts)
@@ -183,7 +209,7 @@ def main():
             "faces": len(host[1]),
             "vertices": len(host[0]),
             "route": "events",
-            "predicates": "fp64",
+            "predicates": "fp32-filtered rays, fp64 fallback",
             "polygons": info["polygons"],
             "triangles": info["triangles"],
             "layers": info["layers"],
@@ -196,9 +222,20 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(args.config, dominant),
             "pipeline_algorithmic_bytes": total,
             "pipeline_GBps": round(total / (timing["median_ms"] * 1e-3) / 1e9, 2),
+        },
+        "fp32_vs_fp64": {
+            args.config: {
+                "fp32_filtered_ms": round(timing["median_ms"], 4),
+                "fp64_ms": round(timing64["median_ms"], 4),
+                "speedup": round(timing64["median_ms"] / timing["median_ms"], 3),
+                "filtered_predicates": info["filtered_predicates"],
+                "fp64_fallbacks": info["fp64_fallbacks"],
+                "fallback_fraction": round(info["fp64_fallbacks"] / decided, 6) if decided else None,
+                "fp64_stages_ms": {k: round(v, 4) for k, v in stages64.items() if k.startswith("k_")},
+            }
         },
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "counters": {k: info[k] for k in ("splits", "ray_groups", "ray_hits", "unique_contours", "rounds",
@@ -206,9 +243,28 @@ def main():
         "cpu_baseline": None,
     }
     del peeler
+    if not args.no_large:
+        name = "nested-shells-717740"
+        large = {}
+        for flag in (False, True):
+            p, h, i, st, t, tot, nv = run_config(name, max(3, args.steps // 2), 1, filter_fp32=flag)
+            large[flag] = (i, t, st)
+            del p
+        (i64, t64, st64), (i32, t32, st32) = large[False], large[True]
+        d32 = i32["filtered_predicates"] + i32["fp64_fallbacks"]
+        result["fp32_vs_fp64"][name] = {
+            "fp32_filtered_ms": round(t32["median_ms"], 3),
+            "fp64_ms": round(t64["median_ms"], 3),
+            "speedup": round(t64["median_ms"] / t32["median_ms"], 3),
+            "layers": i32["layers"],
+            "polygons": i32["polygons"],
+            "triangles": i32["triangles"],
+            "fallback_fraction": round(i32["fp64_fallbacks"] / d32, 6) if d32 else None,
+            "stages_ms": {k: round(v, 3) for k, v in st32.items() if k.startswith("k_")},
+        }
     if args.extras:
         extra = {}
-        for name in ("nested-shells-717740", "stack-65536"):
+        for name in ("stack-65536",):
             p, h, i, st, t, tot, nv = run_config(name, max(3, args.steps // 2), 1)
             extra[name] = {"median_ms": round(t["median_ms"], 3), "layers": i["layers"],
                            "polygons": i["polygons"], "triangles": i["triangles"],

@@ -171,6 +171,9 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   ALLOC(w.leaf_xy, 8L * w.source_tree.size);
   ALLOC(w.leaf_n, w.source_tree.size);
   ALLOC(w.leaf_score, w.source_tree.size);
+  w.filter32 = (d->flags & PEEL_FILTER_FP32) != 0;
+  ALLOC(w.leaf_xy32, w.filter32 ? 8L * w.source_tree.size : 1);
+  ALLOC(w.leaf_tol, w.filter32 ? w.source_tree.size : 1);
   long F = layered ? 1 : w.cap.frontier, R = w.cap.regions;
   // layered route: ping-pong residual pools sized by the frontier capacity,
   // per-worker piece pools, per-layer statistics
@@ -258,6 +261,7 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   w.sort.max_blocks = (int)((sort_cap + RS_TILE - 1) / RS_TILE);
   ALLOC(w.sort.hist, 256L * w.sort.max_blocks);
   ALLOC(w.sort.flags, RS_MAX_PASSES + 1);
+  ALLOC(w.sort.totals, 256);
   long scan_cap = std::max<long>(std::max<long>(R, BV), w.nseg);
   w.scan.max_blocks = (int)((scan_cap + 4095) / 4096) + 1;
   ALLOC(w.scan.block_sums, w.scan.max_blocks);

@@ -52,9 +52,8 @@ __global__ void k_connect_reset(Ws w) {
 }
 
 __global__ void k_patch_counts(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.cap.regions) return;
-  w.vcounts[i] = (i < npolys_c(w) && w.status[ST_ERR] == 0) ? w.out_n[i] : 0;
+  bool ok = w.status[ST_ERR] == 0;
+  ITEM_LOOP(i, npolys_c(w)) w.vcounts[i] = ok ? w.out_n[i] : 0;
 }
 
 __global__ void k_patch_edges(Ws w) {
@@ -72,9 +71,14 @@ __global__ void k_eslots_reset(Ws w) {
 }
 
 // edge_owners (components.py:80-120)
+__device__ void edge_owners_one(const Ws &w, int i);
+
 __global__ void k_edge_owners(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npolys_c(w) || !connecting(w)) return;
+  if (!connecting(w)) return;
+  ITEM_LOOP(i, npolys_c(w)) edge_owners_one(w, i);
+}
+
+__device__ void edge_owners_one(const Ws &w, int i) {
   int n = w.out_n[i];
   int base = w.voffsets[i];
   if (base + n > w.cap.vertices) return;
@@ -116,9 +120,16 @@ __device__ __forceinline__ uint64_t point_key(double2 x) {
 }
 
 // match_edges (components.py:123-159)
+__device__ void match_one(const Ws &w, int i, int n);
+
 __global__ __launch_bounds__(256) void k_match_edges(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool active = i < w.control[C_PEDGES] && i < w.cap.vertices && connecting(w);
+  if (!connecting(w)) return;
+  int n = pedges(w);
+  WAVE_LOOP(base, n) match_one(w, base + threadIdx.x, n);
+}
+
+__device__ void match_one(const Ws &w, int i, int n) {
+  bool active = i < n;
   bool matched = false, missed = false;
   if (active) {
     double2 a = w.starts[i], b = w.ends[i];
@@ -175,9 +186,16 @@ __device__ void unite(int *parents, int a, int b) {
 }
 
 // connect (components.py:162-243)
+__device__ void connect_one(const Ws &w, int i, int n);
+
 __global__ __launch_bounds__(256) void k_connect(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool active = i < w.control[C_PEDGES] && i < w.cap.vertices && connecting(w);
+  if (!connecting(w)) return;
+  int n = pedges(w);
+  WAVE_LOOP(base, n) connect_one(w, base + threadIdx.x, n);
+}
+
+__device__ void connect_one(const Ws &w, int i, int n) {
+  bool active = i < n;
   int other = active ? w.partners[i] : -1;
   active = active && other > i;
   long long visits = 0;
@@ -225,15 +243,15 @@ __global__ __launch_bounds__(256) void k_connect(Ws w) {
 
 // roots (components.py:246-258)
 __global__ __launch_bounds__(256) void k_roots(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool active = i < npolys_c(w) && connecting(w);
-  bool root = false;
-  if (active) {
+  if (!connecting(w)) return;
+  int n = npolys_c(w);
+  long long roots = 0;
+  ITEM_LOOP(i, n) {
     int r = find_root(w.parents, i);
     w.reps[i] = r;
-    root = r == i;
+    roots += r == i;
   }
-  wave_add_metric(&w.metrics[M_COMPONENTS], root);
+  wave_add_metric(&w.metrics[M_COMPONENTS], roots);
 }
 
 static inline dim3 grid_for(long n, int block = 256) {
@@ -244,16 +262,16 @@ static inline dim3 grid_for(long n, int block = 256) {
 void enqueue_connect(const Ws &w, hipStream_t s) {
   long g = ((long)w.cap.regions + 255) / 256;
   hipLaunchKernelGGL(k_connect_reset, dim3((unsigned)(g > 1024 ? 1024 : g)), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_patch_counts, grid_for(w.cap.regions), dim3(256), 0, s, w);
-  scan_exclusive(s, w.vcounts, w.voffsets, nullptr, w.cap.regions, w.cap.regions,
+  hipLaunchKernelGGL(k_patch_counts, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  scan_exclusive(s, w.vcounts, w.voffsets, w.status + ST_POLYS, 0, w.cap.regions,
                  w.control + C_PEDGES, w.scan);
   hipLaunchKernelGGL(k_patch_edges, dim3(1), dim3(1), 0, s, w);
   long eg = ((long)std::max(w.ehash2_size, w.cap.vertices) + 255) / 256;
   hipLaunchKernelGGL(k_eslots_reset, dim3((unsigned)(eg > 1024 ? 1024 : eg)), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_edge_owners, grid_for(w.cap.regions), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_match_edges, grid_for(w.cap.vertices), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_connect, grid_for(w.cap.vertices), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_roots, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_edge_owners, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_match_edges, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_connect, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_roots, stride_grid(w.cap.regions), dim3(256), 0, s, w);
 }
 
 }  // namespace peel

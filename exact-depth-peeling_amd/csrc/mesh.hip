@@ -22,17 +22,19 @@ __device__ __forceinline__ int npolys_m(const Ws &w) {
 }
 
 __global__ void k_vertex_counts(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) w.control[C_CROWDED] = 0;
-  if (i >= w.cap.regions) return;
-  int c = 0;
-  if (i < npolys_m(w) && w.status[ST_ERR] == 0) c = w.out_n[i];
-  w.vcounts[i] = c;
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.control[C_CROWDED] = 0;
+  bool ok = w.status[ST_ERR] == 0;
+  ITEM_LOOP(i, npolys_m(w)) w.vcounts[i] = ok ? w.out_n[i] : 0;
 }
 
+__device__ void scatter_one(const Ws &w, int i);
+
 __global__ void k_scatter(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npolys_m(w) || w.status[ST_ERR] != 0) return;
+  if (w.status[ST_ERR] != 0) return;
+  ITEM_LOOP(i, npolys_m(w)) scatter_one(w, i);
+}
+
+__device__ void scatter_one(const Ws &w, int i) {
   int n = w.out_n[i];
   int base = w.voffsets[i];
   if (n == 0) return;
@@ -72,9 +74,14 @@ __global__ void k_phash_reset(Ws w) {
   for (int j = i; j < size; j += gridDim.x * blockDim.x) w.phash[j] = -1;
 }
 
+__device__ void dedupe_one(const Ws &w, int i);
+
 __global__ void k_point_dedupe(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= load_i(w.status + ST_BVERTS) || w.status[ST_ERR] != 0) return;
+  if (w.status[ST_ERR] != 0) return;
+  ITEM_LOOP(i, load_i(w.status + ST_BVERTS)) dedupe_one(w, i);
+}
+
+__device__ void dedupe_one(const Ws &w, int i) {
   double2 x = w.bverts[i];
   int layer = w.blayers[i];
   int size = phash_live(w), mask = size - 1;
@@ -108,16 +115,19 @@ __device__ __forceinline__ bool first_copy(const Ws &w, int i) {
 }
 
 __global__ void k_point_flags(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.cap.vertices) return;
-  w.pflag[i] = i < load_i(w.status + ST_BVERTS) && w.status[ST_ERR] == 0 && first_copy(w, i);
+  bool ok = w.status[ST_ERR] == 0;
+  ITEM_LOOP(i, load_i(w.status + ST_BVERTS)) w.pflag[i] = ok && first_copy(w, i);
 }
 
 // Unique boundary vertices, compacted, with point boxes (z = layer) for the
 // layered hierarchy (point_bounds, kernels.py:975-991).
+__device__ void compact_one(const Ws &w, int i);
+
 __global__ void k_point_compact(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.cap.vertices || !w.pflag[i]) return;
+  ITEM_LOOP(i, load_i(w.status + ST_BVERTS)) if (w.pflag[i]) compact_one(w, i);
+}
+
+__device__ void compact_one(const Ws &w, int i) {
   int k = w.poff[i];
   double2 p = w.bverts[i];
   float l = (float)w.blayers[i];
@@ -303,9 +313,16 @@ __device__ __forceinline__ bool edge_setup(const Ws &w, int j, EdgeQuery &q, dou
 // more is queued for k_mesh_crowded. Phase 1 counts the fan triangles, one
 // atomic per wave reserves the wave's slots, phase 2 writes uv and lifted
 // world corners.
+__device__ void mesh_edge_one(const Ws &w, int j, int n);
+
 __global__ __launch_bounds__(256) void k_mesh_edges(Ws w) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = j < load_i(w.status + ST_BVERTS) && w.status[ST_ERR] == 0;
+  if (w.status[ST_ERR] != 0) return;
+  int n = load_i(w.status + ST_BVERTS);
+  WAVE_LOOP(base, n) mesh_edge_one(w, base + threadIdx.x, n);
+}
+
+__device__ void mesh_edge_one(const Ws &w, int j, int n) {
+  bool valid = j < n;
   EdgeQuery q;
   double2 center = make_double2(0, 0);
   int i = 0, layer = 0, count = 0;
@@ -469,26 +486,26 @@ __global__ void k_check_vertices(Ws w) {
 }
 
 void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t) {
-  hipLaunchKernelGGL(k_vertex_counts, grid_for(w.cap.regions), dim3(256), 0, s, w);
-  scan_exclusive(s, w.vcounts, w.voffsets, nullptr, w.cap.regions, w.cap.regions,
-                 w.control + C_NBV, w.scan);
+  hipLaunchKernelGGL(k_vertex_counts, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  scan_exclusive(s, w.vcounts, w.voffsets, w.status + ST_POLYS, 0, w.cap.regions, w.control + C_NBV,
+                 w.scan);
   hipLaunchKernelGGL(k_check_vertices, dim3(1), dim3(1), 0, s, w);
-  hipLaunchKernelGGL(k_scatter, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_scatter, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   {
     long g = ((long)w.phash_size + 255) / 256;
     hipLaunchKernelGGL(k_phash_reset, dim3((unsigned)(g > 1024 ? 1024 : g)), dim3(256), 0, s, w);
   }
-  hipLaunchKernelGGL(k_point_dedupe, grid_for(w.cap.vertices), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_point_dedupe, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "scatter_vertices");
-  hipLaunchKernelGGL(k_point_flags, grid_for(w.cap.vertices), dim3(256), 0, s, w);
-  scan_exclusive(s, w.pflag, w.poff, nullptr, w.cap.vertices, w.cap.vertices, w.control + C_NUP,
+  hipLaunchKernelGGL(k_point_flags, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
+  scan_exclusive(s, w.pflag, w.poff, w.status + ST_BVERTS, 0, w.cap.vertices, w.control + C_NUP,
                  w.scan);
-  hipLaunchKernelGGL(k_point_compact, grid_for(w.cap.vertices), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_point_compact, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_point_count, dim3(1), dim3(1), 0, s, w);
   Hierarchy pt = w.point_tree;
   build_hierarchy(s, w.point_boxes, w.control + C_NUP, 0, w.cap.vertices, true, pt, w.sort);
   mark(t, s, "point_hierarchy");
-  hipLaunchKernelGGL(k_mesh_edges, grid_for(w.cap.vertices), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_mesh_edges, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "k_mesh_edges");
   hipLaunchKernelGGL(k_mesh_crowded, dim3(1024), dim3(256), 0, s, w);
   mark(t, s, "k_mesh_crowded");

@@ -101,6 +101,7 @@ enum Control : int {
   C_CROWDED = 138,  // edges with more noding candidates than the register path
   C_BEAM_JOBS = 139, // cells whose certification continues in subtree workers
   C_LOOP = 140,      // layered route: another peeling pass is due
+  C_LEADERS = 141,   // ray-group leaders (they sort first in the ray order)
   C_COUNT = 256,
 };
 
@@ -159,6 +160,37 @@ __device__ __forceinline__ double unkey(uint64_t k) {
 __device__ __forceinline__ void atomic_min_key(unsigned long long *p, double x) {
   atomicMin(p, (unsigned long long)dkey(x));
 }
+
+// Static forward error bound of a binary32 orientation test
+//   s32 = (b.x - a.x) * (x.y - a.y) - (b.y - a.y) * (x.x - a.x)
+// evaluated on binary64 inputs rounded to nearest binary32, when every
+// coordinate is at most M in magnitude and every difference taken is at most
+// D. With u = 2^-24, each rounded difference is off by at most 2uM + uD, each
+// product by (2uM + uD) 2D + uD^2, and the final subtraction adds u|s32|, so
+// |s32 - s| <= 8uMD + 8uD^2 to first order (s: the exact value). The bound
+// doubles that and adds 1e-37 for flushed subnormal products; the binary64
+// evaluation of the same expression is within ~4 2^-53 D^2, far inside it. A
+// filter value beyond the bound therefore has the sign, and a nonzero value,
+// that the binary64 predicate computes. Rounded up to binary32.
+__device__ __forceinline__ float orientation_tol32(double D, double M) {
+  const double u = 5.9604644775390625e-8;  // 2^-24
+  double t = 16.0 * u * D * (M + D) + 1.0e-37;
+  return __double2float_ru(t);
+}
+
+// Kernels over pools whose live count is known only on the device (polygons,
+// patch edges, boundary vertices, hits) launch a capped grid and stride over
+// the live count, instead of one mostly idle lane per capacity slot.
+// WAVE_LOOP keeps every lane of a block in the loop for the same trips, so
+// wave collectives in the body see all 64 lanes (lanes past the count run the
+// body as inactive); ITEM_LOOP is for bodies without wave collectives.
+constexpr int STRIDE_BLOCKS = 8192;  // 32 blocks of 256 lanes per CU
+inline dim3 stride_grid(long items, int block = 256) {
+  long g = (items + block - 1) / block;
+  return dim3((unsigned)(g < 1 ? 1 : (g > STRIDE_BLOCKS ? STRIDE_BLOCKS : g)));
+}
+#define ITEM_LOOP(i, n) for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+#define WAVE_LOOP(base, n) for (int base = blockIdx.x * blockDim.x; base < (n); base += gridDim.x * blockDim.x)
 
 // Live size of an open-addressing table: the power of two at least twice the
 // live key count (load <= 1/2), at least 256, at most the allocated power of

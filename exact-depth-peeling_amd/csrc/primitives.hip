@@ -1,11 +1,13 @@
 // LSD radix sort, exclusive scan and the Morton hierarchy for gfx950.
 //
 // Radix sort: 8-bit digits, 256-thread blocks (four wave64s) over 4096-key
-// tiles. A pass is three kernels: per-tile digit histograms in LDS, one
-// single-block scan of the digit-major histogram matrix, and a stable scatter
-// whose in-tile ranks come from wave64 ballots (eight ballots give each lane
-// the mask of lanes holding the same digit) plus per-wave LDS digit counters.
-// The scan kernel marks a pass trivial when one digit holds every key; the
+// tiles. A pass is three kernels: per-tile digit histograms in LDS; a scan
+// with one block per digit that turns its row of tile counts into offsets
+// within the digit and records the digit's total; and a stable scatter that
+// adds the exclusive scan of the 256 digit totals (computed in LDS by every
+// block) and ranks keys inside the tile with wave64 ballots (eight ballots
+// give each lane the mask of lanes holding the same digit) plus per-wave LDS
+// digit counters. A pass is trivial when one digit holds every key; the
 // scatter then does nothing and the ping-pong parity is tracked on the device,
 // so leading or trailing constant digits cost one histogram pass and no data
 // movement, without any host read.
@@ -24,6 +26,24 @@ __device__ __forceinline__ int live_count(const int *count, int count_static, in
   return n < 0 ? 0 : (n > capacity ? capacity : n);
 }
 
+__device__ __forceinline__ int block_exclusive(int v, int *lds, int &total) {
+  // 256-thread exclusive scan through wave64 shuffles + LDS for the 4 wave sums
+  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) lds[wave] = x;
+  __syncthreads();
+  int prefix = 0;
+  for (int w = 0; w < wave; ++w) prefix += lds[w];
+  total = lds[0] + lds[1] + lds[2] + lds[3];
+  __syncthreads();
+  return prefix + x - v;
+}
+
 // parity of the non-trivial passes before `pass`
 __device__ __forceinline__ int current_buffer(const int *flags, int pass) {
   int sel = 0;
@@ -38,6 +58,8 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist(const uint64_t *k0, const 
   __shared__ int bins[256];
   int n = live_count(count, count_static, capacity);
   int begin = blockIdx.x * RS_TILE;
+  // non-trivial until the scan finds one digit holding every key
+  if (blockIdx.x == 0 && threadIdx.x == 0) flags[pass] = n > 0 ? 1 : 0;
   if (begin >= n) return;
   const uint64_t *keys = current_buffer(flags, pass) ? k1 : k0;
   bins[threadIdx.x] = 0;
@@ -49,43 +71,29 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist(const uint64_t *k0, const 
   hist[threadIdx.x * max_blocks + blockIdx.x] = bins[threadIdx.x];
 }
 
-// One block of 1024 threads: thread t owns digit t/4 and a quarter of the
-// tile columns of that digit.
-__global__ __launch_bounds__(1024) void rs_scan(const int *count, int count_static, int capacity,
-                                                 int pass, int *flags, int *hist, int max_blocks) {
-  __shared__ int sums[1024];
-  __shared__ int trivial;
+// One block per digit: exclusive scan of the digit's tile counts (coalesced
+// chunks of 256) into offsets within the digit, and the digit's total.
+__global__ __launch_bounds__(RS_THREADS) void rs_scan(const int *count, int count_static, int capacity,
+                                                       int pass, int *flags, int *hist, int *totals,
+                                                       int max_blocks) {
+  __shared__ int lds[4];
   int n = live_count(count, count_static, capacity);
   int nb = (n + RS_TILE - 1) / RS_TILE;
-  int t = threadIdx.x, digit = t >> 2, part = t & 3;
-  int per = (nb + 3) / 4;
-  int b0 = min(nb, part * per), b1 = min(nb, b0 + per);
-  int *row = hist + digit * max_blocks;
-  int local = 0;
-  for (int b = b0; b < b1; ++b) local += row[b];
-  sums[t] = local;
-  if (t == 0) trivial = (n == 0);
-  __syncthreads();
-  // digit totals: trivial pass when one digit holds everything
-  if (part == 0) {
-    int total = sums[t] + sums[t + 1] + sums[t + 2] + sums[t + 3];
-    if (total == n) trivial = 1;
+  int digit = blockIdx.x;
+  int *row = hist + (long)digit * max_blocks;
+  int running = 0;
+  for (int c = 0; c < nb; c += RS_THREADS) {
+    int b = c + threadIdx.x;
+    int v = b < nb ? row[b] : 0;
+    int total;
+    int excl = block_exclusive(v, lds, total);
+    if (b < nb) row[b] = running + excl;
+    running += total;
   }
-  // inclusive scan over the 1024 partial sums (Hillis-Steele in LDS)
-  for (int off = 1; off < 1024; off <<= 1) {
-    __syncthreads();
-    int v = t >= off ? sums[t - off] : 0;
-    __syncthreads();
-    sums[t] += v;
+  if (threadIdx.x == 0) {
+    totals[digit] = running;
+    if (n > 0 && running == n) flags[pass] = 0;
   }
-  __syncthreads();
-  int running = sums[t] - local;
-  for (int b = b0; b < b1; ++b) {
-    int c = row[b];
-    row[b] = running;
-    running += c;
-  }
-  if (t == 0) flags[pass] = trivial ? 0 : 1;
 }
 
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint64_t *k0, const int *v0,
@@ -93,9 +101,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint64_t *k0, con
                                                           const int *count, int count_static,
                                                           int capacity, int shift, int pass,
                                                           const int *flags, const int *hist,
-                                                          int max_blocks) {
+                                                          const int *totals, int max_blocks) {
   __shared__ int wave_count[4][256];
   __shared__ int digit_base[256];
+  __shared__ int lds[4];
   int n = live_count(count, count_static, capacity);
   int tile = blockIdx.x * RS_TILE;
   if (tile >= n || flags[pass] == 0) return;
@@ -106,7 +115,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint64_t *k0, con
   int *vout = sel ? (int *)v0 : v1;
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 4 * 256; i += RS_THREADS) (&wave_count[0][0])[i] = 0;
-  digit_base[threadIdx.x] = hist[threadIdx.x * max_blocks + blockIdx.x];
+  int all;
+  int digit_offset = block_exclusive(totals[threadIdx.x], lds, all);
+  digit_base[threadIdx.x] = digit_offset + hist[threadIdx.x * max_blocks + blockIdx.x];
   __syncthreads();
   volatile int *counters = wave_count[wave];
   int base = tile + wave * 64 * RS_ITEMS;
@@ -179,11 +190,11 @@ void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int
     hipLaunchKernelGGL(rs_hist, dim3(blocks), dim3(RS_THREADS), 0, s, keys, alt_keys, count,
                        count_static, capacity, shift, passes, scratch.flags, scratch.hist,
                        scratch.max_blocks);
-    hipLaunchKernelGGL(rs_scan, dim3(1), dim3(1024), 0, s, count, count_static, capacity, passes,
-                       scratch.flags, scratch.hist, scratch.max_blocks);
+    hipLaunchKernelGGL(rs_scan, dim3(256), dim3(RS_THREADS), 0, s, count, count_static, capacity, passes,
+                       scratch.flags, scratch.hist, scratch.totals, scratch.max_blocks);
     hipLaunchKernelGGL(rs_scatter, dim3(blocks), dim3(RS_THREADS), 0, s, keys, ids, alt_keys,
                        alt_ids, count, count_static, capacity, shift, passes, scratch.flags,
-                       scratch.hist, scratch.max_blocks);
+                       scratch.hist, scratch.totals, scratch.max_blocks);
   }
   int grid = blocks * 4 < 4096 ? blocks * 4 : 4096;
   hipLaunchKernelGGL(rs_finish, dim3(grid), dim3(256), 0, s, keys, ids, alt_keys, alt_ids, count,
@@ -194,23 +205,6 @@ void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int
 // Exclusive scan: tile sums, one-block scan of tile sums, tile scans.
 constexpr int SC_THREADS = 256, SC_ITEMS = 16, SC_TILE = SC_THREADS * SC_ITEMS;
 
-__device__ __forceinline__ int block_exclusive(int v, int *lds, int &total) {
-  // 256-thread exclusive scan through wave64 shuffles + LDS for the 4 wave sums
-  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int x = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    int y = __shfl_up(x, off, 64);
-    if (lane >= off) x += y;
-  }
-  if (lane == 63) lds[wave] = x;
-  __syncthreads();
-  int prefix = 0;
-  for (int w = 0; w < wave; ++w) prefix += lds[w];
-  total = lds[0] + lds[1] + lds[2] + lds[3];
-  __syncthreads();
-  return prefix + x - v;
-}
 
 __global__ __launch_bounds__(SC_THREADS) void sc_reduce(const int *in, const int *count,
                                                          int count_static, int capacity,

@@ -14,6 +14,7 @@ constexpr int RS_MAX_PASSES = 16;
 struct SortScratch {
   int *hist = nullptr;   // [256 * max_blocks] digit-major counts / offsets
   int *flags = nullptr;  // [RS_MAX_PASSES + 1] non-trivial pass flags
+  int *totals = nullptr; // [256] keys per digit of the current pass
   int max_blocks = 0;
 };
 

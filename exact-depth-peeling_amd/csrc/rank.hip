@@ -48,15 +48,20 @@ __global__ void k_rank_reset(Ws w) {
     w.control[C_HEAVY] = 0;
     w.control[C_MEDIUM] = 0;
     w.control[C_HUGE] = 0;
+    w.control[C_LEADERS] = 0;
   }
 }
 
 // Canonical sample of the representative patch: centroid summed from the
 // lexicographically smallest vertex (rotation invariant).
+__device__ void sample_one(const Ws &w, int i);
+
 __global__ void k_samples(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int P = npolys(w);
-  if (i >= P || w.status[ST_ERR] != 0) return;
+  if (w.status[ST_ERR] != 0) return;
+  ITEM_LOOP(i, npolys(w)) sample_one(w, i);
+}
+
+__device__ void sample_one(const Ws &w, int i) {
   if (!w.propagate_ranks) w.reps[i] = i;
   int r = w.reps[i];
   int n = w.out_n[r];
@@ -101,10 +106,14 @@ __device__ __forceinline__ int beam_cell(double2 x) {
   return (int)cx * BEAM_SIDE + (int)cy;
 }
 
+__device__ void group_one(const Ws &w, int i);
+
 __global__ void k_groups(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int P = npolys(w);
-  if (i >= P || w.status[ST_ERR] != 0) return;
+  if (w.status[ST_ERR] != 0) return;
+  ITEM_LOOP(i, npolys(w)) group_one(w, i);
+}
+
+__device__ void group_one(const Ws &w, int i) {
   double2 x = w.samples[i];
   int size = qhash_live(w), mask = size - 1;
   int slot = (int)(point_hash(x) & (uint64_t)mask);
@@ -236,14 +245,15 @@ __global__ __launch_bounds__(256) void k_certify_heavy(Ws w, int workers) {
 
 // apply (beams.py:192-207): every patch of a certified cell joins its leader
 __global__ __launch_bounds__(256) void k_beam_apply(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool certified = false;
-  if (i < npolys(w) && w.status[ST_ERR] == 0) {
-    int c = beam_cell(w.samples[i]);
-    if (w.beam_valid[c]) {
-      int leader = w.beam_leader[c];
-      w.groups[i] = leader;
-      certified = leader == i;
+  long long certified = 0;
+  if (w.status[ST_ERR] == 0) {
+    ITEM_LOOP(i, npolys(w)) {
+      int c = beam_cell(w.samples[i]);
+      if (w.beam_valid[c]) {
+        int leader = w.beam_leader[c];
+        w.groups[i] = leader;
+        certified += leader == i;
+      }
     }
   }
   wave_add_metric(&w.metrics[M_BEAMS], certified);
@@ -265,6 +275,28 @@ __global__ void k_leaf_sources(Ws w) {
     w.leaf_score[k] = w.scores[id];
   }
   w.leaf_n[k] = n;
+  if (w.filter32) {
+    // binary32 copy of the ring and the static forward error bound of its
+    // orientation tests against any point a ray query can bring to this leaf
+    double lx = 1e300, ly = 1e300, hx = -1e300, hy = -1e300;
+    for (int j = 0; j < n; ++j) {
+      double2 v = w.leaf_xy[8L * k + j];
+      w.leaf_xy32[8L * k + j] = make_float2((float)v.x, (float)v.y);
+      lx = fmin(lx, v.x), ly = fmin(ly, v.y), hx = fmax(hx, v.x), hy = fmax(hy, v.y);
+    }
+    float tol = 0.f;
+    if (n > 0) {
+      // A query point reaches this leaf only if its ray box (x +- 2e-7)
+      // meets the leaf box (ring bounds +- 2e-7): it lies within 4e-7 of the
+      // ring's bounds plus the binary32 roundings of x, of both paddings and
+      // of the box corners (each at most 2^-24 of a coordinate).
+      double Mr = fmax(fmax(fabs(lx), fabs(hx)), fmax(fabs(ly), fabs(hy)));
+      double D = fmax(hx - lx, hy - ly) + 4.0e-7 + 8.0 * 5.9604644775390625e-8 * (Mr + 1.0e-6);
+      double M = Mr + D;
+      tol = orientation_tol32(D, M);
+    }
+    w.leaf_tol[k] = tol;
+  }
 }
 
 // Half-open point-in-convex test (event_kernels.py:131-148): a point on an
@@ -283,6 +315,43 @@ __device__ __forceinline__ bool contains_leaf(const Ws &w, int k, double2 x) {
     a = b;
   }
   return inside;
+}
+
+// The same test with a binary32 filter: an edge whose binary32 orientation
+// exceeds the record's static error bound in magnitude has the sign, and a
+// nonzero value, that binary64 gives; only the remaining edges are evaluated
+// in binary64, with the half-open rule. The decision therefore equals
+// contains_leaf's for every edge.
+__device__ __forceinline__ bool contains_leaf32(const Ws &w, int k, double2 x, float2 x32, int &filtered,
+                                                int &fallbacks) {
+  int n = w.leaf_n[k];
+  const float2 *r = w.leaf_xy32 + 8L * k;
+  const float tol = w.leaf_tol[k];
+  bool inside = n > 0;
+  float2 a = r[0];
+  for (int j = 0; j < n && inside; ++j) {
+    float2 b = r[(j + 1) % n];
+    float s32 = (b.x - a.x) * (x32.y - a.y) - (b.y - a.y) * (x32.x - a.x);
+    if (s32 < -tol) {
+      inside = false;
+      filtered += 1;
+    } else if (s32 > tol) {
+      filtered += 1;
+    } else {
+      fallbacks += 1;
+      double2 A = w.leaf_xy[8L * k + j], B = w.leaf_xy[8L * k + (j + 1) % n];
+      double s = cross3(A, B, x);
+      bool owned = B.y > A.y || (B.y == A.y && B.x < A.x);
+      if (s < 0.0 || (s == 0.0 && !owned)) inside = false;
+    }
+    a = b;
+  }
+  return inside;
+}
+
+__device__ __forceinline__ bool contains_any(const Ws &w, int k, double2 x, float2 x32, int &filtered,
+                                             int &fallbacks) {
+  return w.filter32 ? contains_leaf32(w, k, x, x32, filtered, fallbacks) : contains_leaf(w, k, x);
 }
 
 // store_hit (event_kernels.py:151-169) with one pool-cursor atomic per wave.
@@ -313,34 +382,48 @@ constexpr int HANDOFF = 32;
 // the same hierarchy nodes (the reference sorts its queries by coordinates,
 // events.py:645-678).
 __global__ void k_ray_keys(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npolys(w)) return;
-  uint64_t key = ~0ull;
-  if (w.status[ST_ERR] == 0 && w.groups[i] == i) {
-    double2 x = w.samples[i];
-    key = morton2((float)x.x, (float)x.y);
+  bool ok = w.status[ST_ERR] == 0;
+  long long leaders = 0;
+  ITEM_LOOP(i, npolys(w)) {
+    // 30-bit keys for leaders, all ones for the rest: leaders sort first
+    uint64_t key = ~0ull;
+    if (ok && w.groups[i] == i) {
+      double2 x = w.samples[i];
+      key = morton2((float)x.x, (float)x.y);
+      leaders += 1;
+    }
+    w.ray_keys[i] = key;
+    w.ray_ids[i] = i;
   }
-  w.ray_keys[i] = key;
-  w.ray_ids[i] = i;
+  leaders = wave_sum(leaders);
+  if ((threadIdx.x & 63) == 0 && leaders) atomicAdd(w.control + C_LEADERS, (int)leaders);
 }
 
+__device__ void ray_one(const Ws &w, int t, int n);
+
+// Leaders only: the first C_LEADERS entries of the Morton-sorted ray order.
 __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int P = npolys(w);
-  int i = t < P ? w.ray_ids[t] : 0;
-  bool lead = t < P && w.status[ST_ERR] == 0 && w.groups[i] == i;
+  if (w.status[ST_ERR] != 0) return;
+  int n = load_i(w.control + C_LEADERS);
+  WAVE_LOOP(base, n) ray_one(w, base + threadIdx.x, n);
+}
+
+__device__ void ray_one(const Ws &w, int t, int n) {
+  int i = t < n ? w.ray_ids[t] : 0;
+  bool lead = t < n;
   Tree tree = w.source_tree.tree();
   double2 x = lead ? w.samples[i] : make_double2(0, 0);
   float4 lo, hi;
   ray_box(x, lo, hi);
-  int node = lead ? 1 : 0, count = 0, visits = 0;
+  int node = lead ? 1 : 0, count = 0, visits = 0, filtered = 0, fallbacks = 0;
   bool heavy32 = false;
+  float2 x32 = make_float2((float)x.x, (float)x.y);
   for (int step = 0; step < HANDOFF && node > 0; ++step) {
     int leaf = query_leaf(tree, lo, hi, node);
     if (leaf < 0) break;
     visits += 1;
     if (visits == 32) heavy32 = node > 0;
-    bool hit = contains_leaf(w, leaf, x);
+    bool hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
     store_hits(w, hit, leaf, i, x);
     count += hit;
   }
@@ -355,6 +438,10 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
   wave_add_metric(&w.metrics[M_RAY_GROUPS], lead);
   wave_add_metric(&w.metrics[M_RAY_TESTS], visits);
   wave_add_metric(&w.metrics[M_HEAVY], heavy32);
+  if (w.filter32) {
+    wave_add_metric(&w.metrics[M_FILTER_TESTS], filtered);
+    wave_add_metric(&w.metrics[M_FALLBACKS], fallbacks);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
@@ -370,6 +457,7 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   int width = tree.leaves / parts;
   long total = (long)jobs * parts;
   long long visits = 0;
+  int filtered = 0, fallbacks = 0;
   for (long base = (long)blockIdx.x * blockDim.x; base < total; base += workers) {
     long task = base + threadIdx.x;
     int node = 0, end = 0, i = 0;
@@ -388,17 +476,22 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
     float4 lo, hi;
     ray_box(x, lo, hi);
     int count = 0;
+    float2 x32 = make_float2((float)x.x, (float)x.y);
     while (node > 0 && node != end) {
       int leaf = query_leaf(tree, lo, hi, node, end);
       if (leaf < 0) break;
       visits += 1;
-      bool hit = contains_leaf(w, leaf, x);
+      bool hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
       store_hits(w, hit, leaf, i, x);
       count += hit;
     }
     if (count) atomicAdd(&w.group_counts[i], count);
   }
   wave_add_metric(&w.metrics[M_RAY_TESTS], visits);
+  if (w.filter32) {
+    wave_add_metric(&w.metrics[M_FILTER_TESTS], filtered);
+    wave_add_metric(&w.metrics[M_FALLBACKS], fallbacks);
+  }
 }
 
 __global__ void k_check_hits(Ws w) {
@@ -408,10 +501,13 @@ __global__ void k_check_hits(Ws w) {
 }
 
 // Stable LSD stages: source, then depth, then group => (group, depth, source).
+__device__ void hit_key_one(const Ws &w, int stage, int i);
+
 __global__ void k_hit_keys(Ws w, int stage) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int H = load_i(w.control + C_SORTN);
-  if (i >= H) return;
+  ITEM_LOOP(i, load_i(w.control + C_SORTN)) hit_key_one(w, stage, i);
+}
+
+__device__ void hit_key_one(const Ws &w, int stage, int i) {
   int h = stage == 0 ? i : w.hids[i];
   if (stage == 0) w.hids[i] = i;
   uint64_t key;
@@ -437,18 +533,25 @@ __device__ __forceinline__ bool hit_less(uint64_t ka, int sa, uint64_t kb, int s
 }
 
 __global__ void k_hit_scatter(Ws w) {
-  int h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= load_i(w.control + C_NHITS) || w.status[ST_ERR] != 0) return;
-  int g = w.hit_group[h];
-  int pos = w.group_offsets[g] + atomicAdd(&w.group_fill[g], 1);
-  w.hids_alt[pos] = h;
+  if (w.status[ST_ERR] != 0) return;
+  ITEM_LOOP(h, load_i(w.control + C_NHITS)) {
+    int g = w.hit_group[h];
+    int pos = w.group_offsets[g] + atomicAdd(&w.group_fill[g], 1);
+    w.hids_alt[pos] = h;
+  }
 }
 
+__device__ void sort_small_one(const Ws &w, int g);
+
+// One wave per ray group (leaders are the first C_LEADERS sorted rays).
 __global__ __launch_bounds__(256) void k_sort_small(Ws w) {
+  if (w.status[ST_ERR] != 0) return;
+  int n = load_i(w.control + C_LEADERS);
+  for (int t = blockIdx.x * 4 + (threadIdx.x >> 6); t < n; t += gridDim.x * 4) sort_small_one(w, w.ray_ids[t]);
+}
+
+__device__ void sort_small_one(const Ws &w, int g) {
   int lane = threadIdx.x & 63;
-  int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (g >= npolys(w) || w.status[ST_ERR] != 0) return;
-  if (w.groups[g] != g) return;
   int c = w.group_counts[g];
   if (c == 0) return;
   if (c > SMALL_SEGMENT) {
@@ -546,10 +649,10 @@ __global__ void k_sort_plan(Ws w) {
 __device__ int assign_one(const Ws &w, int i);
 
 __global__ __launch_bounds__(256) void k_assign(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int P = npolys(w);
   int top = 0;
-  if (i < P && w.status[ST_ERR] == 0) top = assign_one(w, i);
+  if (w.status[ST_ERR] == 0) {
+    ITEM_LOOP(i, npolys(w)) top = max(top, assign_one(w, i));
+  }
   top = wave_max(top);
   if ((threadIdx.x & 63) == 0 && top > 0) atomicMax(w.status + ST_LAYERS, top);
 }
@@ -596,26 +699,26 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   if (w.propagate_ranks) enqueue_connect(w, s);
   mark(t, s, "connectivity");
   hipLaunchKernelGGL(k_leaf_sources, grid_for(w.source_tree.size), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_samples, grid_for(w.cap.regions), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_groups, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_samples, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_groups, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   if (w.share_regions) {
     hipLaunchKernelGGL(k_certify, dim3(BEAM_CELLS / 256), dim3(256), 0, s, w);
     hipLaunchKernelGGL(k_certify_heavy, dim3(32), dim3(256), 0, s, w, 32 * 256);
-    hipLaunchKernelGGL(k_beam_apply, grid_for(w.cap.regions), dim3(256), 0, s, w);
+    hipLaunchKernelGGL(k_beam_apply, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   }
-  hipLaunchKernelGGL(k_ray_keys, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_ray_keys, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   radix_sort(s, w.ray_keys, w.ray_ids, w.ray_keys_alt, w.ray_ids_alt, w.status + ST_POLYS, 0,
              w.cap.regions, 0, 32, w.sort);
   mark(t, s, "samples_groups");
-  hipLaunchKernelGGL(k_ray_hits, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_ray_hits, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   mark(t, s, "k_ray_hits");
   hipLaunchKernelGGL(k_heavy_hits, dim3(256), dim3(256), 0, s, w, 256 * 256);
   mark(t, s, "k_heavy_hits");
   hipLaunchKernelGGL(k_check_hits, dim3(1), dim3(1), 0, s, w);
   scan_exclusive(s, w.group_counts, w.group_offsets, w.status + ST_POLYS, 0, w.cap.regions,
                  nullptr, w.scan);
-  hipLaunchKernelGGL(k_hit_scatter, grid_for(w.cap.hits), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_sort_small, grid_for(w.cap.regions, 4), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_hit_scatter, stride_grid(w.cap.hits), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_sort_small, stride_grid(w.cap.regions, 4), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_sort_medium, dim3(1024), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(1), 0, s, w);
   const int *nh = w.control + C_SORTN;
@@ -623,12 +726,12 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   while ((1L << src_bits) < (long)w.nf) ++src_bits;
   while ((1L << group_bits) < (long)w.cap.regions) ++group_bits;
   for (int stage = 0; stage < 3; ++stage) {
-    hipLaunchKernelGGL(k_hit_keys, grid_for(w.cap.hits), dim3(256), 0, s, w, stage);
+    hipLaunchKernelGGL(k_hit_keys, stride_grid(w.cap.hits), dim3(256), 0, s, w, stage);
     int bits = stage == 0 ? src_bits : (stage == 1 ? 64 : group_bits);
     radix_sort(s, w.hkeys, w.hids, w.hkeys_alt, w.hids_alt, nh, 0, w.cap.hits, 0, bits, w.sort);
   }
   mark(t, s, "hit_sort");
-  hipLaunchKernelGGL(k_assign, grid_for(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_assign, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   mark(t, s, "k_assign");
 }
 

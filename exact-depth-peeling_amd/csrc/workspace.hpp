@@ -67,6 +67,9 @@ struct Ws {
   double2 *leaf_xy;        // [source_tree.size][8] rings in leaf order
   int *leaf_n;             // [source_tree.size]
   double4 *leaf_score;     // [source_tree.size]
+  int filter32;            // binary32-filtered orientation tests (PEEL_FILTER_FP32)
+  float2 *leaf_xy32;       // [source_tree.size][8] binary32 copies of the rings
+  float *leaf_tol;         // [source_tree.size] their static error bounds
 
   // frontier pool [MAXV][frontier]
   double2 *fr_xy;

@@ -57,7 +57,13 @@ class EventCapacities:
 
 
 class EventPeeler(_Workspace):
-    """warp_peel.EventPeeler interface. Requires resolved surface intersections."""
+    """warp_peel.EventPeeler interface. Requires resolved surface intersections.
+
+    ``filter_fp32`` (an extension; the reference has no such switch) decides
+    the rays' point-in-polygon orientation tests in binary32 under a static
+    forward error bound and evaluates only uncertain tests in binary64; the
+    outputs are bit-identical to the binary64 route, and ``read_status``
+    reports ``filtered_predicates`` and ``fp64_fallbacks``."""
 
     _route = _lib.ROUTE_EVENTS
 
@@ -73,6 +79,7 @@ class EventPeeler(_Workspace):
         share_regions=True,
         propagate_ranks=True,
         merge_regions=False,
+        filter_fp32=False,
     ):
         n = faces.shape[0] if faces.ndim == 2 else faces.numel() // 3
         ec = capacities or EventCapacities.for_mesh(n)
@@ -86,10 +93,12 @@ class EventPeeler(_Workspace):
         self.share_regions = share_regions
         self.propagate_ranks = propagate_ranks
         self.merge_regions = merge_regions
+        self.filter_fp32 = filter_fp32
         flags = (
             (_lib.SHARE_REGIONS if share_regions else 0)
             | (_lib.PROPAGATE_RANKS if propagate_ranks else 0)
             | (_lib.MERGE_REGIONS if merge_regions else 0)
+            | (_lib.FILTER_FP32 if filter_fp32 else 0)
         )
         caps = (
             ec.frontier,

@@ -49,13 +49,14 @@ def gpu_outputs(p):
     )
 
 
-def compare(host, threads=8, propagate=True, share=True):
+def compare(host, threads=8, propagate=True, share=True, filter_fp32=False):
     import torch  # noqa: F401
 
     from hip_peel import EventPeeler
     from hip_peel.io import upload
 
-    p = EventPeeler(*upload(*host[:4]), propagate_ranks=propagate, share_regions=share)
+    p = EventPeeler(*upload(*host[:4]), propagate_ranks=propagate, share_regions=share,
+                    filter_fp32=filter_fp32)
     p.run()
     info, rings, srcs, lays, uv, world, tsrc, tlay = gpu_outputs(p)
     cpu = oracle.cpu_event_peel(*host[:4], threads=threads, propagate_ranks=propagate, share_regions=share)
@@ -111,4 +112,43 @@ def test_stack_65536_closed_form():
     assert info["polygons"] == n and info["layers"] == n and info["triangles"] == 3 * n
     assert info["splits"] == 0 and info["unique_contours"] == 3
     assert info["ray_groups"] == 1 and info["ray_hits"] == n and info["ray_candidate_tests"] == n
+    np.testing.assert_array_equal(p.output_layers[:n].cpu().numpy(), p.output_sources[:n].cpu().numpy())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_fixture_fp32_filter_is_bit_identical(name):
+    """binary32-filtered ray containment gives the binary64 route's output bit
+    for bit (checked against the CPU restatement) and counts its decisions."""
+    print(f"[parity] {name} fp32 filter", flush=True)
+    p, info = compare(oracle.load_scene(fixture_path(name))[:4], filter_fp32=True)
+    if info["ray_candidate_tests"]:
+        assert info["filtered_predicates"] + info["fp64_fallbacks"] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grid", [(64, 32), (256, 128)])
+def test_torus_knot_fp32_filter_bit_parity(grid):
+    from hip_peel import scenes
+
+    print(f"[parity] torus knot {grid} fp32 filter", flush=True)
+    p, info = compare(scenes.torus_knot(segments=grid[0], sides=grid[1]), filter_fp32=True)
+    total = info["filtered_predicates"] + info["fp64_fallbacks"]
+    assert total > 0 and info["fp64_fallbacks"] < total
+
+
+@pytest.mark.gpu
+def test_stack_fp32_filter_closed_form():
+    """Depth gaps of 2^-20 do not touch the 2D filter: the stack's footprint
+    tests are decided in binary32 or fall back, and the closed form holds."""
+    from hip_peel import EventPeeler, scenes
+    from hip_peel.io import upload
+
+    print("[parity] stack 65536 fp32 filter", flush=True)
+    n = 65536
+    p = EventPeeler(*upload(*scenes.stack(n)), filter_fp32=True)
+    p.run()
+    info = p.read_status()
+    assert info["polygons"] == n and info["layers"] == n and info["triangles"] == 3 * n
+    assert info["ray_hits"] == n and info["filtered_predicates"] + info["fp64_fallbacks"] >= 3 * n
     np.testing.assert_array_equal(p.output_layers[:n].cpu().numpy(), p.output_sources[:n].cpu().numpy())
