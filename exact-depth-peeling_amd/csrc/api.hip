@@ -163,8 +163,7 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   ALLOC(w.cseg_b, w.nseg);
   ALLOC(w.cnear, 2L * w.nseg);
   int r;
-  if ((r = alloc_hierarchy(ws, w.source_tree, nf1)) || (r = alloc_hierarchy(ws, w.contour_tree, w.nseg)) ||
-      (r = alloc_hierarchy(ws, w.point_tree, w.cap.vertices))) {
+  if ((r = alloc_hierarchy(ws, w.source_tree, nf1)) || (r = alloc_hierarchy(ws, w.contour_tree, w.nseg))) {
     peel_destroy(ws);
     return r;
   }
@@ -243,12 +242,11 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   ALLOC(w.blayers, BV);
   ALLOC(w.tasks, BV);
   ALLOC(w.crowded, BV);
-  ALLOC(w.point_boxes, BV);
-  ALLOC(w.pflag, BV);
-  ALLOC(w.poff, BV);
-  ALLOC(w.upts, BV);
   w.phash_size = pow2_at_least(2 * (int)std::min<long>(BV, 1L << 29));
   ALLOC(w.phash, w.phash_size);
+  ALLOC(w.cell_keys, w.phash_size);
+  ALLOC(w.cell_heads, w.phash_size);
+  ALLOC(w.cell_next, BV);
   ALLOC(w.tri_uv, T * 6);
   ALLOC(w.tri_world, T * 9);
   ALLOC(w.tri_src, T);
@@ -256,7 +254,6 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   long sort_cap = H;
   sort_cap = std::max<long>(sort_cap, w.source_tree.size);
   sort_cap = std::max<long>(sort_cap, w.contour_tree.size);
-  sort_cap = std::max<long>(sort_cap, w.point_tree.size);
   sort_cap = std::max<long>(sort_cap, R);
   w.sort.max_blocks = (int)((sort_cap + RS_TILE - 1) / RS_TILE);
   ALLOC(w.sort.hist, 256L * w.sort.max_blocks);

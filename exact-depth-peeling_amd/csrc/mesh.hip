@@ -4,14 +4,15 @@
 //  vertex_counts / scatter  warp_peel/kernels.py:941-972
 //  mesh_tasks               warp_peel/event_kernels.py:504-516
 //  point_bounds + layered   warp_peel/kernels.py:975-991, events.py:744-754
+//                           (replaced by a per-layer cell grid of vertices)
 //  lift                     warp_peel/kernels.py:994-1006
 //  mesh_one_edge            warp_peel/kernels.py:1011-1124
 //
-// One lane per polygon edge: a point-hierarchy query collects same-layer
-// boundary vertices strictly inside the edge (t in (EPS, 1-EPS), distance
-// <= EPS * length), sorts their parameters, and fans the noded edge from the
-// polygon centroid. Parameters live in a small register window; an edge with
-// more than the window goes through a per-lane spill area in HBM.
+// One lane per polygon edge: a walk over the grid cells along the edge
+// collects same-layer boundary vertices strictly inside the edge (t in
+// (EPS, 1-EPS), distance <= EPS * length), sorts their parameters, and fans
+// the noded edge from the polygon centroid. Up to 8 parameters stay in
+// registers; an edge with more, or a long one, is walked by a whole wave.
 #include "workspace.hpp"
 
 namespace peel {
@@ -52,7 +53,7 @@ __device__ void scatter_one(const Ws &w, int i) {
 }
 
 // Boundary vertices repeat once per incident polygon; only the first copy of
-// each exact (layer, x, y) enters the point hierarchy. Duplicates would give
+// each exact (layer, x, y) enters the vertex grid. Duplicates would give
 // the same edge parameter, which the fan loop skips anyway, so the output is
 // unchanged and each edge query sees each distinct vertex once. The
 // edge_vertices capacity therefore bounds distinct vertices per edge, where
@@ -114,31 +115,123 @@ __device__ __forceinline__ bool first_copy(const Ws &w, int i) {
   return true;
 }
 
-__global__ void k_point_flags(Ws w) {
-  bool ok = w.status[ST_ERR] == 0;
-  ITEM_LOOP(i, load_i(w.status + ST_BVERTS)) w.pflag[i] = ok && first_copy(w, i);
+// ---------------------------------------------------------------------------
+// Cell grid of the distinct boundary vertices, per layer: G x G cells over the
+// unit window (G a power of two chosen on the device for about one distinct
+// vertex per cell), a hash table from (layer, cell) to the head of the cell's
+// vertex list, and one link per vertex. Built in two kernels without sorting;
+// an edge query walks only the cells its segment passes through (instead of a
+// layered hierarchy over vertex boxes, which a long diagonal edge meets over
+// its whole bounding box).
+__device__ __forceinline__ int cells_live(const Ws &w) { return phash_live(w); }
+
+__device__ __forceinline__ uint64_t cell_key(int layer, int cx, int cy) {
+  return ((uint64_t)(uint32_t)layer << 32) | ((uint64_t)(uint32_t)cy << 16) | (uint64_t)(uint32_t)cx;
 }
 
-// Unique boundary vertices, compacted, with point boxes (z = layer) for the
-// layered hierarchy (point_bounds, kernels.py:975-991).
-__device__ void compact_one(const Ws &w, int i);
-
-__global__ void k_point_compact(Ws w) {
-  ITEM_LOOP(i, load_i(w.status + ST_BVERTS)) if (w.pflag[i]) compact_one(w, i);
+__device__ __forceinline__ int cell_index(double v, int G) {
+  double c = floor(v * (double)G);
+  return c < 0.0 ? 0 : (c > (double)(G - 1) ? G - 1 : (int)c);
 }
 
-__device__ void compact_one(const Ws &w, int i) {
-  int k = w.poff[i];
-  double2 p = w.bverts[i];
-  float l = (float)w.blayers[i];
-  w.upts[k] = p;
-  Box b;
-  b.lo = make_float4((float)p.x - 2.0e-7f, (float)p.y - 2.0e-7f, l, 0.f);
-  b.hi = make_float4((float)p.x + 2.0e-7f, (float)p.y + 2.0e-7f, l, 0.f);
-  w.point_boxes[k] = b;
+// G: power of two with G^2 about the distinct vertices per layer (copies are
+// about twice the distinct vertices), between 8 and 4096.
+__global__ void k_grid_plan(Ws w) {
+  double per_layer = (double)load_i(w.status + ST_BVERTS) / (2.0 * (double)max(1, w.status[ST_LAYERS]));
+  int G = 8;
+  while (G < 4096 && (double)G * (double)G < per_layer) G *= 2;
+  w.control[C_GRID] = G;
 }
 
-__global__ void k_point_count(Ws w) { w.metrics[M_UNIQUE_BVERTS] = w.control[C_NUP]; }
+__global__ void k_cells_reset(Ws w) {
+  int size = cells_live(w);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < size; j += gridDim.x * blockDim.x) {
+    w.cell_keys[j] = ~0ull;
+    w.cell_heads[j] = -1;
+  }
+}
+
+__device__ void cell_insert_one(const Ws &w, int i, int G);
+
+// Link every first copy of a (layer, point) into its cell's list.
+__global__ void k_cell_insert(Ws w) {
+  long long unique = 0;
+  if (w.status[ST_ERR] == 0) {
+    int G = load_i(w.control + C_GRID);
+    ITEM_LOOP(i, load_i(w.status + ST_BVERTS)) {
+      if (first_copy(w, i)) {
+        cell_insert_one(w, i, G);
+        unique += 1;
+      }
+    }
+  }
+  wave_add_metric(&w.metrics[M_UNIQUE_BVERTS], unique);
+}
+
+__device__ void cell_insert_one(const Ws &w, int i, int G) {
+  double2 x = w.bverts[i];
+  uint64_t key = cell_key(w.blayers[i], cell_index(x.x, G), cell_index(x.y, G));
+  int size = cells_live(w), mask = size - 1;
+  int slot = (int)(mix64(key) & (uint64_t)mask);
+  for (int probe = 0; probe < size; ++probe) {
+    unsigned long long cur = __hip_atomic_load(w.cell_keys + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == ~0ull) cur = atomicCAS(w.cell_keys + slot, ~0ull, (unsigned long long)key);
+    if (cur == ~0ull || cur == key) {
+      w.cell_next[i] = atomicExch(w.cell_heads + slot, i);
+      return;
+    }
+    slot = (slot + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ int cell_head(const Ws &w, int layer, int cx, int cy, int size) {
+  uint64_t key = cell_key(layer, cx, cy);
+  int mask = size - 1;
+  int slot = (int)(mix64(key) & (uint64_t)mask);
+  for (int probe = 0; probe < size; ++probe) {
+    unsigned long long cur = w.cell_keys[slot];
+    if (cur == key) return w.cell_heads[slot];
+    if (cur == ~0ull) return -1;
+    slot = (slot + 1) & mask;
+  }
+  return -1;
+}
+
+// Visit every distinct vertex of `layer` in a cell that can hold a point within
+// PAD of the segment p0-p1: each column the padded x-range touches and, in it,
+// each row that the segment's y-range over the padded column (itself padded)
+// touches. A vertex accepted by the edge test lies within EPS of the segment,
+// far inside PAD, so it is visited; each cell, and so each vertex, once.
+constexpr double NODE_PAD = 2.0e-7;
+
+template <class Visit>
+__device__ __forceinline__ void walk_segment(const Ws &w, double2 p0, double2 p1, int layer, int G, int size,
+                                             Visit visit) {
+  double xa = fmin(p0.x, p1.x), xb = fmax(p0.x, p1.x);
+  double ya = fmin(p0.y, p1.y), yb = fmax(p0.y, p1.y);
+  double dx = p1.x - p0.x, dy = p1.y - p0.y;
+  int cx0 = cell_index(xa - NODE_PAD, G), cx1 = cell_index(xb + NODE_PAD, G);
+  for (int cx = cx0; cx <= cx1; ++cx) {
+    double xl = fmax(xa, (double)cx / G - NODE_PAD), xr = fmin(xb, (double)(cx + 1) / G + NODE_PAD);
+    double ylo = ya, yhi = yb;
+    if (dx != 0.0 && xl <= xr) {
+      double y0 = p0.y + (xl - p0.x) * (dy / dx), y1 = p0.y + (xr - p0.x) * (dy / dx);
+      ylo = fmax(ya, fmin(y0, y1));
+      yhi = fmin(yb, fmax(y0, y1));
+    }
+    int cy0 = cell_index(ylo - NODE_PAD, G), cy1 = cell_index(yhi + NODE_PAD, G);
+    for (int cy = cy0; cy <= cy1; ++cy)
+      for (int k = cell_head(w, layer, cx, cy, size); k >= 0; k = w.cell_next[k])
+        if (!visit(k)) return;
+  }
+}
+
+// Cells of the segment's padded bounding box, a bound on the cells walked.
+__device__ __forceinline__ int box_cells(double2 a, double2 b, int G) {
+  int cols = cell_index(fmax(a.x, b.x) + NODE_PAD, G) - cell_index(fmin(a.x, b.x) - NODE_PAD, G) + 1;
+  int rows = cell_index(fmax(a.y, b.y) + NODE_PAD, G) - cell_index(fmin(a.y, b.y) - NODE_PAD, G) + 1;
+  return cols + rows;
+}
 
 __device__ __forceinline__ void lift(const Ws &w, double2 x, double4 sc, double *out) {
   const double *s = w.settings;
@@ -156,42 +249,11 @@ __device__ __forceinline__ void lift(const Ws &w, double2 x, double4 sc, double 
 
 constexpr int EDGE_MAX = 128;
 
-// Heapsort in place (kernels.py:1066-1088); Array is any indexable store.
-template <class Array>
-__device__ void heapsort(Array &params, int count) {
-  int start = count / 2 - 1, end = count - 1;
-  while (end > 0) {
-    int root = 0;
-    if (start >= 0) {
-      root = start;
-      start -= 1;
-    } else {
-      double t = params[end];
-      params[end] = params[0];
-      params[0] = t;
-      end -= 1;
-    }
-    while (root * 2 + 1 <= end) {
-      int child = root * 2 + 1;
-      if (child + 1 <= end && params[child] < params[child + 1]) child += 1;
-      if (params[root] < params[child]) {
-        double t = params[root];
-        params[root] = params[child];
-        params[child] = t;
-        root = child;
-      } else {
-        break;
-      }
-    }
-  }
-}
-
 // Parameters of the same-layer vertices strictly inside edge (a, b):
 // t in (EPS, 1 - EPS) and distance <= EPS * |b - a| (kernels.py:1040-1064).
 struct EdgeQuery {
   double2 a, b, d;
   double length2, len;
-  float4 lo, hi;
   __device__ bool accept(double2 x, double &t) const {
     t = ((x.x - a.x) * d.x + (x.y - a.y) * d.y) / length2;
     return t > EPS && t < 1.0 - EPS && fabs(cross3(a, b, x)) <= EPS * len;
@@ -199,6 +261,7 @@ struct EdgeQuery {
 };
 
 constexpr int REG_PARAMS = 8;
+constexpr int LONG_EDGE_CELLS = 48;  // grid columns + rows beyond which an edge is walked by a wave
 
 // Sorting network on 8 registers (Batcher odd-even merge), static indices only.
 __device__ __forceinline__ void cswap(double &x, double &y) {
@@ -214,34 +277,6 @@ __device__ __forceinline__ void sort8(double (&p)[REG_PARAMS]) {
   cswap(p[1], p[4]); cswap(p[3], p[6]);
   cswap(p[2], p[4]); cswap(p[3], p[5]);
   cswap(p[3], p[4]);
-}
-
-// Per-lane parameter array in LDS, strided by the block width so the lanes of
-// a wave touch consecutive banks.
-constexpr int CROWD_THREADS = 64;
-struct LdsParams {
-  double *base;  // &lds[0][lane]
-  __device__ double &operator[](int k) { return base[k * CROWD_THREADS]; }
-  __device__ double operator[](int k) const { return base[k * CROWD_THREADS]; }
-};
-
-// Crowded edges (more than REG_PARAMS interior vertices): gather up to
-// edge_vertices parameters and heapsort them, as the reference does for every
-// edge.
-__device__ int gather_crowded(const Ws &w, const EdgeQuery &q, LdsParams &params) {
-  Tree tree = w.point_tree.tree();
-  int node = 1, count = 0;
-  while (node > 0) {
-    int k = query_next(tree, q.lo, q.hi, node);
-    if (k < 0) break;
-    double t;
-    if (q.accept(w.upts[k], t)) {
-      if (count < w.cap.edge_vertices) params[count] = t;
-      count += 1;
-    }
-  }
-  if (count <= w.cap.edge_vertices) heapsort(params, count);
-  return count;
 }
 
 // Fan walk over the sorted parameters: counts (emit == false) or writes
@@ -303,8 +338,6 @@ __device__ __forceinline__ bool edge_setup(const Ws &w, int j, EdgeQuery &q, dou
     center.y += v.y / (double)n;
   }
   layer = w.out_layer[i];
-  q.lo = make_float4((float)fmin(q.a.x, q.b.x) - 2.0e-7f, (float)fmin(q.a.y, q.b.y) - 2.0e-7f, (float)layer, 0.f);
-  q.hi = make_float4((float)fmax(q.a.x, q.b.x) + 2.0e-7f, (float)fmax(q.a.y, q.b.y) + 2.0e-7f, (float)layer, 0.f);
   return true;
 }
 
@@ -331,22 +364,24 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
 #pragma unroll
   for (int k = 0; k < REG_PARAMS; ++k) p[k] = 2.0;
   if (valid) valid = edge_setup(w, j, q, center, i, layer);
-  if (valid) {
-    Tree tree = w.point_tree.tree();
-    int node = q.length2 > 0.0 ? 1 : 0;
-    while (node > 0) {
-      int k = query_next(tree, q.lo, q.hi, node);
-      if (k < 0) break;
-      double t;
-      if (q.accept(w.upts[k], t)) {
+  if (valid && q.length2 > 0.0) {
+    const int G = load_i(w.control + C_GRID), size = cells_live(w);
+    if (box_cells(q.a, q.b, G) > LONG_EDGE_CELLS) {
+      crowded = true;  // long edges go straight to the wave-parallel path
+    } else {
+      walk_segment(w, q.a, q.b, layer, G, size, [&](int k) {
+        double t;
+        if (q.accept(w.bverts[k], t)) {
 #pragma unroll
-        for (int r = 0; r < REG_PARAMS; ++r) p[r] = (r == count) ? t : p[r];
-        count += 1;
-        if (count > REG_PARAMS) {
-          crowded = true;
-          break;
+          for (int r = 0; r < REG_PARAMS; ++r) p[r] = (r == count) ? t : p[r];
+          count += 1;
+          if (count > REG_PARAMS) {
+            crowded = true;
+            return false;
+          }
         }
-      }
+        return true;
+      });
     }
     sort8(p);
   }
@@ -404,17 +439,12 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
     double t0 = lane / 64.0, t1 = (lane + 1) / 64.0;
     double2 p0 = make_double2(q.a.x + t0 * q.d.x, q.a.y + t0 * q.d.y);
     double2 p1 = make_double2(q.a.x + t1 * q.d.x, q.a.y + t1 * q.d.y);
-    float4 lo = make_float4((float)fmin(p0.x, p1.x) - 2.0e-7f, (float)fmin(p0.y, p1.y) - 2.0e-7f, q.lo.z, 0.f);
-    float4 hi = make_float4((float)fmax(p0.x, p1.x) + 2.0e-7f, (float)fmax(p0.y, p1.y) + 2.0e-7f, q.hi.z, 0.f);
+    const int G = load_i(w.control + C_GRID), size = cells_live(w);
     double mine[LANE_PARAMS];
     int count = 0;
-    Tree tree = w.point_tree.tree();
-    int node = q.length2 > 0.0 ? 1 : 0;
-    while (node > 0) {
-      int k = query_next(tree, lo, hi, node);
-      if (k < 0) break;
+    if (q.length2 > 0.0) walk_segment(w, p0, p1, layer, G, size, [&](int k) {
       double tt;
-      if (q.accept(w.upts[k], tt) && tt >= t0 && (tt < t1 || lane == 63)) {
+      if (q.accept(w.bverts[k], tt) && tt >= t0 && (tt < t1 || lane == 63)) {
         // insertion into the lane's sorted run (static indices only)
         if (count < LANE_PARAMS) {
 #pragma unroll
@@ -429,7 +459,8 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
         }
         count += 1;
       }
-    }
+      return true;
+    });
     int total;
     int offset = wave_prefix(count, total);
     if (lane == 0) atomicMax((unsigned long long *)&w.metrics[M_EDGE_MAX], (unsigned long long)total);
@@ -443,13 +474,12 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
     if (count > LANE_PARAMS) {
       // a lane with a dense sub-range gathers again straight into LDS and
       // insertion-sorts its run there
-      int node2 = 1, m = 0;
-      while (node2 > 0) {
-        int k = query_next(tree, lo, hi, node2);
-        if (k < 0) break;
+      int m = 0;
+      walk_segment(w, p0, p1, layer, G, size, [&](int k) {
         double tt;
-        if (q.accept(w.upts[k], tt) && tt >= t0 && (tt < t1 || lane == 63)) list[offset + m++] = tt;
-      }
+        if (q.accept(w.bverts[k], tt) && tt >= t0 && (tt < t1 || lane == 63)) list[offset + m++] = tt;
+        return true;
+      });
       for (int a = 1; a < m; ++a) {
         double v = list[offset + a];
         int b = a - 1;
@@ -497,14 +527,13 @@ void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t) {
   }
   hipLaunchKernelGGL(k_point_dedupe, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "scatter_vertices");
-  hipLaunchKernelGGL(k_point_flags, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
-  scan_exclusive(s, w.pflag, w.poff, w.status + ST_BVERTS, 0, w.cap.vertices, w.control + C_NUP,
-                 w.scan);
-  hipLaunchKernelGGL(k_point_compact, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_point_count, dim3(1), dim3(1), 0, s, w);
-  Hierarchy pt = w.point_tree;
-  build_hierarchy(s, w.point_boxes, w.control + C_NUP, 0, w.cap.vertices, true, pt, w.sort);
-  mark(t, s, "point_hierarchy");
+  hipLaunchKernelGGL(k_grid_plan, dim3(1), dim3(1), 0, s, w);
+  {
+    long g = ((long)w.phash_size + 255) / 256;
+    hipLaunchKernelGGL(k_cells_reset, dim3((unsigned)(g > 1024 ? 1024 : g)), dim3(256), 0, s, w);
+  }
+  hipLaunchKernelGGL(k_cell_insert, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
+  mark(t, s, "vertex_grid");
   hipLaunchKernelGGL(k_mesh_edges, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "k_mesh_edges");
   hipLaunchKernelGGL(k_mesh_crowded, dim3(1024), dim3(256), 0, s, w);

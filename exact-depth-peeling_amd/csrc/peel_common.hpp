@@ -94,7 +94,6 @@ enum Control : int {
   C_MEDIUM = 131,   // groups sorted by a workgroup
   C_HUGE = 132,     // any group too large for a workgroup sort
   C_NSEG = 133,     // unique contour segments
-  C_NUP = 134,      // unique boundary vertices
   C_INTERNAL = 135, // any internal (non-contour) mesh edge
   C_CONNECT = 136,  // connectivity pass enabled for this run
   C_PEDGES = 137,   // patch edges (scan total)
@@ -102,6 +101,7 @@ enum Control : int {
   C_BEAM_JOBS = 139, // cells whose certification continues in subtree workers
   C_LOOP = 140,      // layered route: another peeling pass is due
   C_LEADERS = 141,   // ray-group leaders (they sort first in the ray order)
+  C_GRID = 142,      // cells per axis of the boundary-vertex grid
   C_COUNT = 256,
 };
 

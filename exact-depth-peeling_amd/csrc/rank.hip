@@ -372,11 +372,31 @@ __device__ __forceinline__ void ray_box(double2 x, float4 &lo, float4 &hi) {
   hi = make_float4((float)x.x + 2.0e-7f, (float)x.y + 2.0e-7f, 1.0e30f, 0.f);
 }
 
-// A lane keeps its ray for up to HANDOFF candidates; only longer queries hand
-// their unvisited depth-first suffix to the subtree-parallel kernel. The
-// reference counts a group as heavy once it passes 32 candidates
-// (event_kernels.py:222-245); that counter keeps its meaning.
+// K lanes per ray, chosen on the device from the number of rays: 64 when at
+// most 8,192 rays would leave most of the chip idle (65,536-face torus knot:
+// 4,116 rays, ray stage 0.90 -> 0.42 ms), 1 when there are more (717,740-face
+// shells: one lane per ray keeps 64 Morton-adjacent rays per wave walking the
+// same nodes; 4 lanes per ray measured 2.5x slower there). The K lanes of a ray split the source
+// hierarchy into its K subtrees at depth log2 K and each walks its own, so a
+// ray's candidates are visited in parallel and lanes whose subtree box misses
+// the ray stop after one node. A lane that reaches HANDOFF candidates hands the
+// unvisited rest of its subtree to the subtree-parallel kernel (or keeps
+// walking if that queue is full). The reference counts a group as heavy once
+// it passes 32 candidates (event_kernels.py:222-245): that counter keeps its
+// meaning.
 constexpr int HANDOFF = 32;
+constexpr long RAY_LANES_TARGET = 524288;  // lanes in flight that fill 256 CUs
+
+__device__ __forceinline__ int ray_lanes(const Tree &tree, int rays) {
+  if ((long)64 * rays > RAY_LANES_TARGET) return 1;
+  return tree.leaves < 64 ? tree.leaves : 64;
+}
+
+// Sum over the aligned group of K lanes (every lane of the wave active).
+__device__ __forceinline__ long long group_sum(long long v, int k) {
+  for (int off = k >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
 
 // Ray leaders in Morton order of their samples, so the lanes of a wave walk
 // the same hierarchy nodes (the reference sorts its queries by coordinates,
@@ -399,62 +419,86 @@ __global__ void k_ray_keys(Ws w) {
   if ((threadIdx.x & 63) == 0 && leaders) atomicAdd(w.control + C_LEADERS, (int)leaders);
 }
 
-__device__ void ray_one(const Ws &w, int t, int n);
-
-// Leaders only: the first C_LEADERS entries of the Morton-sorted ray order.
+// ray_hits (event_kernels.py:196-245), leaders only: the first C_LEADERS
+// entries of the Morton-sorted ray order, K lanes each.
 __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
   if (w.status[ST_ERR] != 0) return;
-  int n = load_i(w.control + C_LEADERS);
-  WAVE_LOOP(base, n) ray_one(w, base + threadIdx.x, n);
-}
-
-__device__ void ray_one(const Ws &w, int t, int n) {
-  int i = t < n ? w.ray_ids[t] : 0;
-  bool lead = t < n;
-  Tree tree = w.source_tree.tree();
-  double2 x = lead ? w.samples[i] : make_double2(0, 0);
-  float4 lo, hi;
-  ray_box(x, lo, hi);
-  int node = lead ? 1 : 0, count = 0, visits = 0, filtered = 0, fallbacks = 0;
-  bool heavy32 = false;
-  float2 x32 = make_float2((float)x.x, (float)x.y);
-  for (int step = 0; step < HANDOFF && node > 0; ++step) {
-    int leaf = query_leaf(tree, lo, hi, node);
-    if (leaf < 0) break;
-    visits += 1;
-    if (visits == 32) heavy32 = node > 0;
-    bool hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
-    store_hits(w, hit, leaf, i, x);
-    count += hit;
+  const int n = load_i(w.control + C_LEADERS);
+  const int lane = threadIdx.x & 63;
+  const Tree tree = w.source_tree.tree();
+  const int K = ray_lanes(tree, n);
+  const int sub = lane & (K - 1);
+  const int per_wave = 64 / K;
+  const long waves = (gridDim.x * blockDim.x) >> 6;
+  long long groups = 0, visits_all = 0, heavy = 0;
+  int filtered = 0, fallbacks = 0;
+  for (long base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * per_wave; base < n; base += waves * per_wave) {
+    long r = base + lane / K;
+    bool active = r < n;
+    int i = active ? w.ray_ids[r] : 0;
+    double2 x = active ? w.samples[i] : make_double2(0, 0);
+    float2 x32 = make_float2((float)x.x, (float)x.y);
+    float4 lo, hi;
+    ray_box(x, lo, hi);
+    const int root = K + sub;
+    int node = active ? root : 0;
+    const int end = successor(root);
+    int count = 0, visits = 0;
+    bool handed = false, queue_full = false;
+    while (node > 0 && node != end) {
+      if (visits >= HANDOFF && !queue_full) {
+        // the unvisited suffix of this subtree starts at the first leaf
+        // below the cursor
+        int first = node;
+        while (first < tree.leaves) first *= 2;
+        int slot = wave_claim32(w.control + C_HEAVY, true);
+        if (slot < w.cap.regions) {
+          w.heavy[slot] = make_int4(i, first, root, 0);
+          handed = true;
+          break;
+        }
+        queue_full = true;
+      }
+      int leaf = query_leaf(tree, lo, hi, node, end);
+      if (leaf < 0) break;
+      visits += 1;
+      bool hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
+      store_hits(w, hit, leaf, i, x);
+      count += hit;
+    }
+    long long total = group_sum(count, K);
+    long long tested = group_sum(visits, K);
+    long long any_handed = group_sum(handed ? 1 : 0, K);
+    if (active && sub == 0) {
+      w.group_counts[i] = (int)total;
+      groups += 1;
+      visits_all += tested;
+      heavy += (any_handed > 0 || tested > 32) ? 1 : 0;
+    }
   }
-  if (lead) w.group_counts[i] = count;
-  bool handoff = lead && node > 0;
-  if (handoff) {
-    // the unvisited suffix starts at the first leaf below the cursor
-    while (node < tree.leaves) node *= 2;
-  }
-  int slot = wave_claim32(w.control + C_HEAVY, handoff);
-  if (handoff) w.heavy[slot] = make_int2(i, node);
-  wave_add_metric(&w.metrics[M_RAY_GROUPS], lead);
-  wave_add_metric(&w.metrics[M_RAY_TESTS], visits);
-  wave_add_metric(&w.metrics[M_HEAVY], heavy32);
+  wave_add_metric(&w.metrics[M_RAY_GROUPS], groups);
+  wave_add_metric(&w.metrics[M_RAY_TESTS], visits_all);
+  wave_add_metric(&w.metrics[M_HEAVY], heavy);
   if (w.filter32) {
     wave_add_metric(&w.metrics[M_FILTER_TESTS], filtered);
     wave_add_metric(&w.metrics[M_FALLBACKS], fallbacks);
   }
 }
 
+// heavy_ray_hits (event_kernels.py:248-298): each handed-off job (ray, first
+// unvisited leaf, lane subtree) is split into aligned parts of its subtree,
+// sized so jobs * parts stays near the worker count; parts before the cursor
+// are skipped and the part holding it starts there.
 __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   bool ok = w.status[ST_ERR] == 0;
   Tree tree = w.source_tree.tree();
   int jobs = ok ? load_i(w.control + C_HEAVY) : 0;
-  // Subtree partition of each job's unvisited suffix (event_kernels.py:258-262),
-  // sized so jobs * parts stays near the worker count; with many jobs each one
-  // simply continues from its cursor (parts = 1) instead of re-entering 16
-  // subtrees from their roots.
-  int parts = min(8192, tree.leaves);
+  if (jobs > w.cap.regions) jobs = w.cap.regions;
+  // leaves per lane subtree, with the lanes per ray that k_ray_hits chose
+  const int W = tree.leaves / ray_lanes(tree, load_i(w.control + C_LEADERS));
+  int parts = W;
   while (parts > 1 && (long long)parts * jobs > 65536) parts /= 2;
-  int width = tree.leaves / parts;
+  const int width = W / parts;
   long total = (long)jobs * parts;
   long long visits = 0;
   int filtered = 0, fallbacks = 0;
@@ -462,13 +506,13 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
     long task = base + threadIdx.x;
     int node = 0, end = 0, i = 0;
     if (task < total) {
-      int2 job = w.heavy[task / parts];
+      int4 job = w.heavy[task / parts];
       int part = (int)(task % parts);
-      int first = tree.leaves + part * width;
+      int part_root = job.z * parts + part;
+      int first = part_root * width;  // first leaf node of the part
       if (job.y < first + width) {
-        int root = parts + part;
-        node = job.y > first ? job.y : root;
-        end = successor(root);
+        node = job.y > first ? job.y : part_root;
+        end = successor(part_root);
         i = job.x;
       }
     }

@@ -63,7 +63,7 @@ struct Ws {
   double *cnear;           // [2][nseg] their minimum endpoint depths
 
   // hierarchies
-  Hierarchy source_tree, contour_tree, point_tree;
+  Hierarchy source_tree, contour_tree;
   double2 *leaf_xy;        // [source_tree.size][8] rings in leaf order
   int *leaf_n;             // [source_tree.size]
   double4 *leaf_score;     // [source_tree.size]
@@ -91,7 +91,7 @@ struct Ws {
   uint64_t *ray_keys, *ray_keys_alt;  // [regions] Morton keys of ray samples
   int *ray_ids, *ray_ids_alt;         // [regions] rays in Morton order
   int *medium;             // [regions] groups sorted by one workgroup
-  int2 *heavy;             // [regions]
+  int4 *heavy;             // [regions] (ray, first unvisited leaf, lane subtree root)
   // rectangle certification of shared rays over 32 x 32 window cells
   unsigned long long *beam_box;  // [4][BEAM_CELLS] order-preserving keys: min x, min y, max x, max y
   int *beam_leader;        // [BEAM_CELLS] smallest patch index in the cell
@@ -116,10 +116,10 @@ struct Ws {
   int *blayers;             // [vertices]
   int2 *tasks;              // [vertices]
   int *crowded;             // [vertices] edges for the per-lane-array noding path
-  Box *point_boxes;         // [vertices] boxes of the compacted unique vertices
-  int *pflag, *poff;        // [vertices] first-copy flags and their scan
-  double2 *upts;            // [vertices] unique (layer, point) boundary vertices
   int *phash;               // [phash_size] boundary vertex dedupe (layer, x, y)
+  unsigned long long *cell_keys;  // [phash_size] (layer, cell) keys of the vertex grid
+  int *cell_heads;          // [phash_size] first vertex of each cell's list
+  int *cell_next;           // [vertices] next vertex in the same cell
   int phash_size;
   double *tri_uv;           // [triangles][3][2]
   double *tri_world;        // [triangles][3][3]
