@@ -145,7 +145,8 @@ __device__ void match_one(const Ws &w, int i, int n) {
           missed = false;
           break;
         }
-        __threadfence();
+        // the occupant's owner, task and endpoints were written by
+        // k_edge_owners, a previous kernel: no fence is needed to read them
         if (w.owners[other] == owner && w.tasks[i].x != w.tasks[other].x && same(a, w.ends[other]) &&
             same(b, w.starts[other])) {
           w.partners[i] = other;

@@ -426,9 +426,16 @@ __global__ void k_segments(Ws w) {
   w.seg_a[i] = a;
   w.seg_b[i] = b;
   w.seg_rep[i] = -1;
-  if (!active) return;
-  // publish the endpoints before the slot index becomes visible in the table
-  __threadfence();
+}
+
+// Exact duplicate grouping: the minimum slot of equal segments owns the hash
+// slot. The endpoints compared were written by k_segments (a previous
+// kernel), so the probe needs no fences.
+__global__ void k_segments_insert(Ws w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.nseg) return;
+  double2 a = w.seg_a[i], b = w.seg_b[i];
+  if (a.x >= 1e29) return;
   int mask = w.shash_size - 1;
   int slot = (int)(segment_hash(a, b) & (uint64_t)mask);
   for (int probe = 0; probe < w.shash_size; ++probe) {
@@ -437,7 +444,6 @@ __global__ void k_segments(Ws w) {
       cur = atomicCAS(&w.shash[slot], -1, i);
       if (cur < 0) return;
     }
-    __threadfence();  // acquire: the occupant's endpoints, not a stale L1 copy
     double2 ca = w.seg_a[cur], cb = w.seg_b[cur];
     if (same(ca, a) && same(cb, b)) {
       if (i < cur) atomicMin(&w.shash[slot], i);
@@ -528,6 +534,7 @@ void enqueue_prepare(const Ws &w, hipStream_t s, Timeline *t) {
     return;
   }
   hipLaunchKernelGGL(k_segments, grid_for(w.nseg), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_segments_insert, grid_for(w.nseg), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_segments_reduce, grid_for(w.nseg), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_segment_flags, grid_for(w.nseg), dim3(256), 0, s, w);
   scan_exclusive(s, w.seg_flag, w.seg_off, nullptr, w.nseg, w.nseg, w.control + C_NSEG, w.scan);

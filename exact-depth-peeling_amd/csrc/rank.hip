@@ -84,7 +84,20 @@ __device__ void sample_one(const Ws &w, int i) {
   w.samples[i] = x;
   w.group_counts[i] = 0;
   w.group_fill[i] = 0;
-  __threadfence();
+}
+
+// Exact sample groups: the first patch to claim a sample's slot leads it. The
+// samples compared here were written by k_samples (a previous kernel), so the
+// probe needs no fences.
+__device__ void sample_insert_one(const Ws &w, int i);
+
+__global__ void k_sample_insert(Ws w) {
+  if (w.status[ST_ERR] != 0) return;
+  ITEM_LOOP(i, npolys(w)) sample_insert_one(w, i);
+}
+
+__device__ void sample_insert_one(const Ws &w, int i) {
+  double2 x = w.samples[i];
   int size = qhash_live(w), mask = size - 1;
   int slot = (int)(point_hash(x) & (uint64_t)mask);
   for (int probe = 0; probe < size; ++probe) {
@@ -93,7 +106,6 @@ __device__ void sample_one(const Ws &w, int i) {
       cur = atomicCAS(&w.qhash[slot], -1, i);
       if (cur < 0) return;
     }
-    __threadfence();
     if (same(w.samples[cur], x)) return;
     slot = (slot + 1) & mask;
   }
@@ -744,6 +756,7 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   mark(t, s, "connectivity");
   hipLaunchKernelGGL(k_leaf_sources, grid_for(w.source_tree.size), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_samples, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_sample_insert, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_groups, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   if (w.share_regions) {
     hipLaunchKernelGGL(k_certify, dim3(BEAM_CELLS / 256), dim3(256), 0, s, w);

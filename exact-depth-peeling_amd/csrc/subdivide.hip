@@ -51,12 +51,28 @@ __device__ __forceinline__ Slot slot_of(const Ws &w, int task) {
 __device__ __forceinline__ double2 vtx(const Slot &s, int j) { return s.xy[(long)j * s.stride]; }
 
 
-// A lane's current ring lives in LDS, vertex-major so the 64 lanes of a wave
-// read vertex j of their rings from consecutive banks.
+// The first SUB_LDS vertices of a lane's current ring live in LDS,
+// vertex-major so the 64 lanes of a wave read vertex j of their rings from
+// consecutive banks; the rare longer ring reads the rest from its slot (slots
+// are never reused within a run). 192 B of LDS per lane lets several blocks
+// share a CU.
 constexpr int SUB_THREADS = 256;
+constexpr int SUB_LDS = 12;
+__shared__ double2 sub_ring_lds[SUB_LDS * SUB_THREADS];
+// kept out of line so the compiler does not fold the LDS and global loads
+// into one generic load
+__device__ __noinline__ double2 slot_vertex(const double2 *slot, long stride, int j) {
+  return slot[(long)j * stride];
+}
 struct LdsRing {
-  double2 *base;  // &ring[0][lane]
-  __device__ double2 operator[](int j) const { return base[j * SUB_THREADS]; }
+  int lane;             // this lane's column of sub_ring_lds
+  const double2 *slot;  // the task's slot: vertex j at slot[j * stride]
+  long stride;
+  __device__ double2 operator[](int j) const {
+    if (j < SUB_LDS) return sub_ring_lds[j * SUB_THREADS + lane];
+    return slot_vertex(slot, stride, j);
+  }
+  __device__ void put(int j, double2 v) const { sub_ring_lds[j * SUB_THREADS + lane] = v; }
 };
 
 // frontier.py:81-103
@@ -95,8 +111,7 @@ __device__ void save_output(const Ws &w, const LdsRing &s, int n, int source, in
 }
 
 __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
-  __shared__ double2 ring_lds[MAXV * SUB_THREADS];
-  const LdsRing ring{ring_lds + threadIdx.x};
+  LdsRing ring{(int)threadIdx.x, nullptr, 0};
   const int nf = w.nf;
   const Tree tree = w.contour_tree.tree();
   int lane = threadIdx.x & 63;
@@ -121,13 +136,15 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
       // A ticket past the frontier capacity can never be allocated: it only
       // waits for the pending count to reach zero. Polling is relaxed; the
       // acquire fence runs once, when a published slot is taken.
-      bool ready = ticket < nf || (ticket - nf < w.cap.frontier &&
-                                   load_i(w.fr_ready + (ticket - nf)) != 0);
-      if (__ballot(ready && ticket >= nf)) acquire_fence();
+      int flag = ticket < nf ? 1 : (ticket - nf < w.cap.frontier ? load_i(w.fr_ready + (ticket - nf)) : 0);
+      bool ready = flag != 0;
+      if (__ballot(flag == 1 && ticket >= nf)) acquire_fence();
       if (ready) {
         task = ticket;
         ticket = -1;
-        if (task < nf) {
+        if (flag == 2) {
+          task = -1;  // a left child its cutting lane kept for itself
+        } else if (task < nf) {
           n = w.src_n[task];
           source = task;
           node = 1;
@@ -144,7 +161,9 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
           task = -1;
         } else {
           Slot src = slot_of(w, task);
-          for (int j = 0; j < n && j < MAXV; ++j) ring.base[j * SUB_THREADS] = vtx(src, j);
+          for (int j = 0; j < n && j < SUB_LDS; ++j) ring.put(j, vtx(src, j));
+          ring.slot = src.xy;
+          ring.stride = src.stride;
         }
       }
     }
@@ -245,18 +264,26 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
       w.fr_node[k + 1] = node;
       w.fr_gen[k] = gen + 1;
       w.fr_gen[k + 1] = gen + 1;
-      __threadfence();
-      store_release(w.fr_ready + k, 1);
-      store_release(w.fr_ready + k + 1, 1);
+      // The lane keeps the left child (flag 2: whoever draws that ticket skips
+      // it) and publishes only the right one, after one release.
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(w.fr_ready + k, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(w.fr_ready + k + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       int live = atomicAdd(w.control + C_PENDING, 1) + 1;
       atomicMax(w.status + ST_PEAK, live);
+      task = nf + k;
+      n = left;
+      gen += 1;
+      for (int j = 0; j < n && j < SUB_LDS; ++j) ring.put(j, L[(long)j * fcap]);
+      ring.slot = L;
+      ring.stride = fcap;
       cut = true;
       break;
     }
     bool finish = !cut && done;
     int o = wave_claim32(w.status + ST_POLYS, finish);
     if (cut) {
-      task = -1;
+      // continue with the kept left child in the next visit
     } else if (finish) {
       save_output(w, s, n, source, o);
       atomicSub(w.control + C_PENDING, 1);
