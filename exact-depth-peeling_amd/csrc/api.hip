@@ -125,6 +125,7 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   w.all_layers = d->layers == 0;
   w.share_regions = (d->flags & PEEL_SHARE_REGIONS) != 0;
   w.propagate_ranks = (d->flags & PEEL_PROPAGATE_RANKS) != 0;
+  w.merge_regions = (d->flags & PEEL_MERGE_REGIONS) != 0;
   w.route = d->route;
   const bool layered = d->route == PEEL_ROUTE_LAYERED;
   w.workers = layered ? d->workers : 1;
@@ -234,6 +235,12 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   ALLOC(w.ends, BV);
   ALLOC(w.partners, BV);
   ALLOC(w.parents, R);
+  ALLOC(w.choices, R);
+  long MR = w.merge_regions ? R : 1;
+  ALLOC(w.merge_xy, MR * MAXV);
+  ALLOC(w.merge_n, MR);
+  ALLOC(w.merge_src, MR);
+  ALLOC(w.merge_layer, MR);
   w.ehash2_size = pow2_at_least(2 * (int)std::min<long>(BV, 1L << 29));
   ALLOC(w.eslots, w.ehash2_size);
   ALLOC(w.vcounts, R);
@@ -244,8 +251,8 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   ALLOC(w.crowded, BV);
   w.phash_size = pow2_at_least(2 * (int)std::min<long>(BV, 1L << 29));
   ALLOC(w.phash, w.phash_size);
-  ALLOC(w.cell_keys, w.phash_size);
-  ALLOC(w.cell_heads, w.phash_size);
+  w.cell_capacity = 4 * BV;
+  ALLOC(w.cell_heads, w.cell_capacity);
   ALLOC(w.cell_next, BV);
   ALLOC(w.tri_uv, T * 6);
   ALLOC(w.tri_world, T * 9);

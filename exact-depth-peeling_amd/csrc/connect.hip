@@ -7,6 +7,8 @@
 //  certified connect     components.py:162-243 (midpoint contour query,
 //                        lock-free min-index union-find)
 //  roots / identity      components.py:246-271
+//  convex coalescing     components.py:274-395 (propose, merge, live_counts,
+//                        compact, restore; merge_regions=True)
 //
 // Patch edges are tasks (polygon, edge) at the scanned ring offsets. An edge's
 // owner is the welded key of the internal mesh edge it lies on, or 2^63 |
@@ -42,12 +44,13 @@ __global__ void k_connect_reset(Ws w) {
   for (int j = i; j < P; j += gridDim.x * blockDim.x) {
     w.parents[j] = j;
     w.reps[j] = j;
+    w.choices[j] = 0x7fffffff;
   }
   if (i == 0) {
     // components.py:40-52: connect only after a split or with an internal edge
     int any = (w.metrics[M_SPLITS] > 0) || (w.control[C_INTERNAL] != 0);
     w.control[C_CONNECT] = any;
-    if (!any) w.metrics[M_COMPONENTS] = npolys_c(w);
+    if (!any && w.propagate_ranks) w.metrics[M_COMPONENTS] = npolys_c(w);
   }
 }
 
@@ -271,8 +274,122 @@ void enqueue_connect(const Ws &w, hipStream_t s) {
   hipLaunchKernelGGL(k_eslots_reset, dim3((unsigned)(eg > 1024 ? 1024 : eg)), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_edge_owners, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_match_edges, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_connect, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_roots, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  if (w.propagate_ranks) {
+    hipLaunchKernelGGL(k_connect, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
+    hipLaunchKernelGGL(k_roots, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Convex coalescing (components.py:274-395), after ranks: two matched patch
+// edges of the same source and layer propose joining their patches when both
+// junctions stay convex; each patch keeps its smallest proposal, a pair whose
+// proposals agree is joined (collinear vertices dropped), and the surviving
+// patches are compacted.
+__device__ __forceinline__ double2 ring_vertex(const Ws &w, int p, int j) {
+  return w.out_xy[(long)j * w.cap.regions + p];
+}
+
+__device__ __forceinline__ bool merge_pair(const Ws &w, int i, int &pa, int &pb, int &ka, int &kb) {
+  int other = w.partners[i];
+  if (other <= i) return false;
+  int2 ta = w.tasks[i], tb = w.tasks[other];
+  pa = ta.x, pb = tb.x, ka = ta.y, kb = tb.y;
+  return true;
+}
+
+__global__ void k_propose(Ws w) {
+  if (!connecting(w)) return;
+  ITEM_LOOP(i, pedges(w)) {
+    int pa, pb, ka, kb;
+    if (!merge_pair(w, i, pa, pb, ka, kb)) continue;
+    if (w.out_src[pa] != w.out_src[pb] || w.out_layer[pa] != w.out_layer[pb]) continue;
+    int na = w.out_n[pa], nb = w.out_n[pb];
+    if (na < 3 || nb < 3 || na + nb - 2 > MAXV) continue;
+    double2 u = ring_vertex(w, pa, ka), v = ring_vertex(w, pb, kb);
+    if (cross3(ring_vertex(w, pa, (ka + na - 1) % na), u, ring_vertex(w, pb, (kb + 2) % nb)) >= 0.0 &&
+        cross3(ring_vertex(w, pb, (kb + nb - 1) % nb), v, ring_vertex(w, pa, (ka + 2) % na)) >= 0.0) {
+      atomicMin(w.choices + pa, i);
+      atomicMin(w.choices + pb, i);
+    }
+  }
+}
+
+__global__ void k_merge(Ws w) {
+  if (!connecting(w)) return;
+  long long merged = 0;
+  ITEM_LOOP(i, pedges(w)) {
+    int pa, pb, ka, kb;
+    if (!merge_pair(w, i, pa, pb, ka, kb)) continue;
+    if (w.choices[pa] != i || w.choices[pb] != i) continue;
+    int na = w.out_n[pa], nb = w.out_n[pb];
+    // a's ring from the vertex after the shared edge, then b's vertices
+    // between the shared edge's ends
+    double2 joined[MAXV];
+    int m = 0;
+    for (int j = 0; j < na; ++j) joined[m++] = ring_vertex(w, pa, (ka + 1 + j) % na);
+    for (int j = 0; j < nb - 2; ++j) joined[m++] = ring_vertex(w, pb, (kb + 2 + j) % nb);
+    int q = 0;
+    for (int j = 0; j < m; ++j) {
+      double2 prev = joined[(j + m - 1) % m], x = joined[j], next = joined[(j + 1) % m];
+      if (cross3(prev, x, next) != 0.0) w.out_xy[(long)q++ * w.cap.regions + pa] = x;
+    }
+    // fewer than three vertices left cannot happen for two convex patches of
+    // positive area; the reference keeps both patches unchanged then
+    if (q >= 3) {
+      w.out_n[pa] = q;
+      w.out_n[pb] = 0;
+      merged += 1;
+    }
+  }
+  wave_add_metric(&w.metrics[M_MERGED], merged);
+}
+
+__global__ void k_live_counts(Ws w) {
+  bool on = connecting(w);
+  ITEM_LOOP(i, npolys_c(w)) w.vcounts[i] = on ? (w.out_n[i] > 0) : 1;
+}
+
+__global__ void k_compact(Ws w) {
+  if (!connecting(w)) return;
+  long R = w.cap.regions;
+  ITEM_LOOP(i, npolys_c(w)) {
+    int n = w.out_n[i];
+    if (n == 0) continue;
+    int slot = w.voffsets[i];
+    for (int j = 0; j < n; ++j) w.merge_xy[(long)j * R + slot] = w.out_xy[(long)j * R + i];
+    w.merge_n[slot] = n;
+    w.merge_src[slot] = w.out_src[i];
+    w.merge_layer[slot] = w.out_layer[i];
+  }
+}
+
+__global__ void k_restore(Ws w) {
+  if (!connecting(w)) return;
+  long R = w.cap.regions;
+  int live = load_i(w.control + C_LIVE);
+  ITEM_LOOP(i, live) {
+    int n = w.merge_n[i];
+    for (int j = 0; j < n; ++j) w.out_xy[(long)j * R + i] = w.merge_xy[(long)j * R + i];
+    w.out_n[i] = n;
+    w.out_src[i] = w.merge_src[i];
+    w.out_layer[i] = w.merge_layer[i];
+  }
+}
+
+__global__ void k_live_total(Ws w) {
+  if (connecting(w) && w.status[ST_ERR] == 0) w.status[ST_POLYS] = w.control[C_LIVE];
+}
+
+void enqueue_merge(const Ws &w, hipStream_t s) {
+  hipLaunchKernelGGL(k_propose, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_merge, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_live_counts, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  scan_exclusive(s, w.vcounts, w.voffsets, w.status + ST_POLYS, 0, w.cap.regions, w.control + C_LIVE,
+                 w.scan);
+  hipLaunchKernelGGL(k_compact, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_restore, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_live_total, dim3(1), dim3(1), 0, s, w);
 }
 
 }  // namespace peel

@@ -118,15 +118,13 @@ __device__ __forceinline__ bool first_copy(const Ws &w, int i) {
 // ---------------------------------------------------------------------------
 // Cell grid of the distinct boundary vertices, per layer: G x G cells over the
 // unit window (G a power of two chosen on the device for about one distinct
-// vertex per cell), a hash table from (layer, cell) to the head of the cell's
-// vertex list, and one link per vertex. Built in two kernels without sorting;
-// an edge query walks only the cells its segment passes through (instead of a
-// layered hierarchy over vertex boxes, which a long diagonal edge meets over
-// its whole bounding box).
-__device__ __forceinline__ int cells_live(const Ws &w) { return phash_live(w); }
-
-__device__ __forceinline__ uint64_t cell_key(int layer, int cx, int cy) {
-  return ((uint64_t)(uint32_t)layer << 32) | ((uint64_t)(uint32_t)cy << 16) | (uint64_t)(uint32_t)cx;
+// vertex per cell), a dense array of list heads indexed by (layer, cell), and
+// one link per vertex. Built in two kernels without sorting; an edge query
+// walks only the cells its segment passes through (instead of a layered
+// hierarchy over vertex boxes, which a long diagonal edge meets over its whole
+// bounding box). layers * G^2 stays within the allocated heads.
+__device__ __forceinline__ long cell_slot(int layer, int cx, int cy, int G) {
+  return ((long)layer * G + cy) * G + cx;
 }
 
 __device__ __forceinline__ int cell_index(double v, int G) {
@@ -137,18 +135,22 @@ __device__ __forceinline__ int cell_index(double v, int G) {
 // G: power of two with G^2 about the distinct vertices per layer (copies are
 // about twice the distinct vertices), between 8 and 4096.
 __global__ void k_grid_plan(Ws w) {
-  double per_layer = (double)load_i(w.status + ST_BVERTS) / (2.0 * (double)max(1, w.status[ST_LAYERS]));
+  int layers = max(1, w.status[ST_LAYERS]);
+  double per_layer = (double)load_i(w.status + ST_BVERTS) / (2.0 * (double)layers);
   int G = 8;
   while (G < 4096 && (double)G * (double)G < per_layer) G *= 2;
+  while (G > 1 && (long)layers * G * G > w.cell_capacity) G /= 2;
   w.control[C_GRID] = G;
 }
 
+__device__ __forceinline__ long cells_live(const Ws &w, int G) {
+  return (long)max(1, w.status[ST_LAYERS]) * G * G;
+}
+
 __global__ void k_cells_reset(Ws w) {
-  int size = cells_live(w);
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < size; j += gridDim.x * blockDim.x) {
-    w.cell_keys[j] = ~0ull;
+  long size = cells_live(w, load_i(w.control + C_GRID));
+  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < size; j += (long)gridDim.x * blockDim.x)
     w.cell_heads[j] = -1;
-  }
 }
 
 __device__ void cell_insert_one(const Ws &w, int i, int G);
@@ -170,31 +172,8 @@ __global__ void k_cell_insert(Ws w) {
 
 __device__ void cell_insert_one(const Ws &w, int i, int G) {
   double2 x = w.bverts[i];
-  uint64_t key = cell_key(w.blayers[i], cell_index(x.x, G), cell_index(x.y, G));
-  int size = cells_live(w), mask = size - 1;
-  int slot = (int)(mix64(key) & (uint64_t)mask);
-  for (int probe = 0; probe < size; ++probe) {
-    unsigned long long cur = __hip_atomic_load(w.cell_keys + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (cur == ~0ull) cur = atomicCAS(w.cell_keys + slot, ~0ull, (unsigned long long)key);
-    if (cur == ~0ull || cur == key) {
-      w.cell_next[i] = atomicExch(w.cell_heads + slot, i);
-      return;
-    }
-    slot = (slot + 1) & mask;
-  }
-}
-
-__device__ __forceinline__ int cell_head(const Ws &w, int layer, int cx, int cy, int size) {
-  uint64_t key = cell_key(layer, cx, cy);
-  int mask = size - 1;
-  int slot = (int)(mix64(key) & (uint64_t)mask);
-  for (int probe = 0; probe < size; ++probe) {
-    unsigned long long cur = w.cell_keys[slot];
-    if (cur == key) return w.cell_heads[slot];
-    if (cur == ~0ull) return -1;
-    slot = (slot + 1) & mask;
-  }
-  return -1;
+  long slot = cell_slot(w.blayers[i], cell_index(x.x, G), cell_index(x.y, G), G);
+  w.cell_next[i] = atomicExch(w.cell_heads + slot, i);
 }
 
 // Visit every distinct vertex of `layer` in a cell that can hold a point within
@@ -205,8 +184,7 @@ __device__ __forceinline__ int cell_head(const Ws &w, int layer, int cx, int cy,
 constexpr double NODE_PAD = 2.0e-7;
 
 template <class Visit>
-__device__ __forceinline__ void walk_segment(const Ws &w, double2 p0, double2 p1, int layer, int G, int size,
-                                             Visit visit) {
+__device__ __forceinline__ void walk_segment(const Ws &w, double2 p0, double2 p1, int layer, int G, Visit visit) {
   double xa = fmin(p0.x, p1.x), xb = fmax(p0.x, p1.x);
   double ya = fmin(p0.y, p1.y), yb = fmax(p0.y, p1.y);
   double dx = p1.x - p0.x, dy = p1.y - p0.y;
@@ -221,7 +199,7 @@ __device__ __forceinline__ void walk_segment(const Ws &w, double2 p0, double2 p1
     }
     int cy0 = cell_index(ylo - NODE_PAD, G), cy1 = cell_index(yhi + NODE_PAD, G);
     for (int cy = cy0; cy <= cy1; ++cy)
-      for (int k = cell_head(w, layer, cx, cy, size); k >= 0; k = w.cell_next[k])
+      for (int k = w.cell_heads[cell_slot(layer, cx, cy, G)]; k >= 0; k = w.cell_next[k])
         if (!visit(k)) return;
   }
 }
@@ -365,11 +343,11 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
   for (int k = 0; k < REG_PARAMS; ++k) p[k] = 2.0;
   if (valid) valid = edge_setup(w, j, q, center, i, layer);
   if (valid && q.length2 > 0.0) {
-    const int G = load_i(w.control + C_GRID), size = cells_live(w);
+    const int G = load_i(w.control + C_GRID);
     if (box_cells(q.a, q.b, G) > LONG_EDGE_CELLS) {
       crowded = true;  // long edges go straight to the wave-parallel path
     } else {
-      walk_segment(w, q.a, q.b, layer, G, size, [&](int k) {
+      walk_segment(w, q.a, q.b, layer, G, [&](int k) {
         double t;
         if (q.accept(w.bverts[k], t)) {
 #pragma unroll
@@ -439,10 +417,10 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
     double t0 = lane / 64.0, t1 = (lane + 1) / 64.0;
     double2 p0 = make_double2(q.a.x + t0 * q.d.x, q.a.y + t0 * q.d.y);
     double2 p1 = make_double2(q.a.x + t1 * q.d.x, q.a.y + t1 * q.d.y);
-    const int G = load_i(w.control + C_GRID), size = cells_live(w);
+    const int G = load_i(w.control + C_GRID);
     double mine[LANE_PARAMS];
     int count = 0;
-    if (q.length2 > 0.0) walk_segment(w, p0, p1, layer, G, size, [&](int k) {
+    if (q.length2 > 0.0) walk_segment(w, p0, p1, layer, G, [&](int k) {
       double tt;
       if (q.accept(w.bverts[k], tt) && tt >= t0 && (tt < t1 || lane == 63)) {
         // insertion into the lane's sorted run (static indices only)
@@ -475,7 +453,7 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
       // a lane with a dense sub-range gathers again straight into LDS and
       // insertion-sorts its run there
       int m = 0;
-      walk_segment(w, p0, p1, layer, G, size, [&](int k) {
+      walk_segment(w, p0, p1, layer, G, [&](int k) {
         double tt;
         if (q.accept(w.bverts[k], tt) && tt >= t0 && (tt < t1 || lane == 63)) list[offset + m++] = tt;
         return true;
@@ -492,13 +470,54 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
-    if (lane == 0) {
-      auto mem = [&](int k) { return list[k]; };
-      int source = w.out_src[i];
-      double4 sc = w.scores[source];
-      int ntri = fan(w, q, center, total, mem, false, 0, 0, 0, sc);
-      int base = atomicAdd(w.status + ST_TRIS, ntri);
-      fan(w, q, center, total, mem, true, base, source, layer, sc);
+    int source = w.out_src[i];
+    double4 sc = w.scores[source];
+    // The fan steps run across the lanes when no two consecutive parameters
+    // (0 and 1 included) lie within 2 EPS: then the sequential skip rule keeps
+    // every step, and step s joins the points at parameters s-1 and s, computed
+    // exactly as the sequential walk computes them. Otherwise lane 0 walks.
+    const int steps = total + 1;
+    bool tight = false;
+    for (int st = lane; st < steps; st += 64) {
+      double prev = st == 0 ? 0.0 : list[st - 1], cur = st < total ? list[st] : 1.0;
+      tight = tight || cur <= prev + 2.0 * EPS;
+    }
+    if (__ballot(tight) != 0) {
+      if (lane == 0) {
+        auto mem = [&](int k) { return list[k]; };
+        int ntri = fan(w, q, center, total, mem, false, 0, 0, 0, sc);
+        int base = atomicAdd(w.status + ST_TRIS, ntri);
+        fan(w, q, center, total, mem, true, base, source, layer, sc);
+      }
+    } else {
+      for (int round = 0; round < steps; round += 64) {
+        int st = round + lane;
+        bool emit = false;
+        double2 cursor = q.a, next = q.a;
+        if (st < steps) {
+          double prev = st == 0 ? 0.0 : list[st - 1], cur = st < total ? list[st] : 1.0;
+          if (st > 0) cursor = make_double2(q.a.x + prev * q.d.x, q.a.y + prev * q.d.y);
+          next = make_double2(q.a.x + cur * q.d.x, q.a.y + cur * q.d.y);
+          emit = fabs(cross3(center, cursor, next)) > EPS * EPS;
+        }
+        int slot = wave_claim32(w.status + ST_TRIS, emit);
+        if (emit) {
+          if (slot < w.cap.triangles) {
+            double2 tri[3] = {center, cursor, next};
+            double *uv = w.tri_uv + 6L * slot;
+            double *world = w.tri_world + 9L * slot;
+            for (int c = 0; c < 3; ++c) {
+              uv[2 * c] = tri[c].x;
+              uv[2 * c + 1] = tri[c].y;
+              lift(w, tri[c], sc, world + 3 * c);
+            }
+            w.tri_src[slot] = source;
+            w.tri_layer[slot] = layer;
+          } else {
+            set_error(w.status, ERR_TRIS);
+          }
+        }
+      }
     }
     __builtin_amdgcn_wave_barrier();
   }
@@ -529,7 +548,7 @@ void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t) {
   mark(t, s, "scatter_vertices");
   hipLaunchKernelGGL(k_grid_plan, dim3(1), dim3(1), 0, s, w);
   {
-    long g = ((long)w.phash_size + 255) / 256;
+    long g = (w.cell_capacity + 255) / 256;
     hipLaunchKernelGGL(k_cells_reset, dim3((unsigned)(g > 1024 ? 1024 : g)), dim3(256), 0, s, w);
   }
   hipLaunchKernelGGL(k_cell_insert, stride_grid(w.cap.vertices), dim3(256), 0, s, w);

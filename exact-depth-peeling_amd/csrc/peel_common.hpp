@@ -102,6 +102,7 @@ enum Control : int {
   C_LOOP = 140,      // layered route: another peeling pass is due
   C_LEADERS = 141,   // ray-group leaders (they sort first in the ray order)
   C_GRID = 142,      // cells per axis of the boundary-vertex grid
+  C_LIVE = 143,      // patches left after convex coalescing
   C_COUNT = 256,
 };
 

@@ -499,8 +499,8 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
 
 // heavy_ray_hits (event_kernels.py:248-298): each handed-off job (ray, first
 // unvisited leaf, lane subtree) is split into aligned parts of its subtree,
-// sized so jobs * parts stays near the worker count; parts before the cursor
-// are skipped and the part holding it starts there.
+// sized so jobs * parts stays near 65,536; parts before the cursor are skipped
+// and the part holding it starts there.
 __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   bool ok = w.status[ST_ERR] == 0;
   Tree tree = w.source_tree.tree();
@@ -508,6 +508,8 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   if (jobs > w.cap.regions) jobs = w.cap.regions;
   // leaves per lane subtree, with the lanes per ray that k_ray_hits chose
   const int W = tree.leaves / ray_lanes(tree, load_i(w.control + C_LEADERS));
+  // a job split into more parts re-enters that many subtrees from their
+  // roots: split only while the split jobs still fit 65,536 lanes
   int parts = W;
   while (parts > 1 && (long long)parts * jobs > 65536) parts /= 2;
   const int width = W / parts;
@@ -746,13 +748,11 @@ static inline dim3 grid_for(long n, int block = 256) {
   return dim3((unsigned)(g < 1 ? 1 : g));
 }
 
-void enqueue_connect(const Ws &w, hipStream_t s);
-
 void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   long qgrid = ((long)w.qhash_size + 255) / 256;
   if (qgrid > 1024) qgrid = 1024;
   hipLaunchKernelGGL(k_rank_reset, dim3((unsigned)qgrid), dim3(256), 0, s, w);
-  if (w.propagate_ranks) enqueue_connect(w, s);
+  if (w.propagate_ranks || w.merge_regions) enqueue_connect(w, s);
   mark(t, s, "connectivity");
   hipLaunchKernelGGL(k_leaf_sources, grid_for(w.source_tree.size), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_samples, stride_grid(w.cap.regions), dim3(256), 0, s, w);
@@ -769,7 +769,12 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   mark(t, s, "samples_groups");
   hipLaunchKernelGGL(k_ray_hits, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   mark(t, s, "k_ray_hits");
-  hipLaunchKernelGGL(k_heavy_hits, dim3(256), dim3(256), 0, s, w, 256 * 256);
+  {
+    // enough workers for many handed-off rays (each continues in one lane
+    // when there are more than 65,536 jobs); idle blocks return at once
+    dim3 hg = stride_grid(w.cap.regions);
+    hipLaunchKernelGGL(k_heavy_hits, hg, dim3(256), 0, s, w, (int)hg.x * 256);
+  }
   mark(t, s, "k_heavy_hits");
   hipLaunchKernelGGL(k_check_hits, dim3(1), dim3(1), 0, s, w);
   scan_exclusive(s, w.group_counts, w.group_offsets, w.status + ST_POLYS, 0, w.cap.regions,
@@ -790,6 +795,10 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   mark(t, s, "hit_sort");
   hipLaunchKernelGGL(k_assign, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   mark(t, s, "k_assign");
+  if (w.merge_regions) {
+    enqueue_merge(w, s);
+    mark(t, s, "merge_regions");
+  }
 }
 
 }  // namespace peel

@@ -136,15 +136,13 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
       // A ticket past the frontier capacity can never be allocated: it only
       // waits for the pending count to reach zero. Polling is relaxed; the
       // acquire fence runs once, when a published slot is taken.
-      int flag = ticket < nf ? 1 : (ticket - nf < w.cap.frontier ? load_i(w.fr_ready + (ticket - nf)) : 0);
-      bool ready = flag != 0;
-      if (__ballot(flag == 1 && ticket >= nf)) acquire_fence();
+      bool ready = ticket < nf || (ticket - nf < w.cap.frontier &&
+                                   load_i(w.fr_ready + (ticket - nf)) != 0);
+      if (__ballot(ready && ticket >= nf)) acquire_fence();
       if (ready) {
         task = ticket;
         ticket = -1;
-        if (flag == 2) {
-          task = -1;  // a left child its cutting lane kept for itself
-        } else if (task < nf) {
+        if (task < nf) {
           n = w.src_n[task];
           source = task;
           node = 1;
@@ -264,26 +262,19 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
       w.fr_node[k + 1] = node;
       w.fr_gen[k] = gen + 1;
       w.fr_gen[k + 1] = gen + 1;
-      // The lane keeps the left child (flag 2: whoever draws that ticket skips
-      // it) and publishes only the right one, after one release.
+      // one release for both children, then the two ready flags
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      __hip_atomic_store(w.fr_ready + k, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(w.fr_ready + k, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(w.fr_ready + k + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       int live = atomicAdd(w.control + C_PENDING, 1) + 1;
       atomicMax(w.status + ST_PEAK, live);
-      task = nf + k;
-      n = left;
-      gen += 1;
-      for (int j = 0; j < n && j < SUB_LDS; ++j) ring.put(j, L[(long)j * fcap]);
-      ring.slot = L;
-      ring.stride = fcap;
       cut = true;
       break;
     }
     bool finish = !cut && done;
     int o = wave_claim32(w.status + ST_POLYS, finish);
     if (cut) {
-      // continue with the kept left child in the next visit
+      task = -1;
     } else if (finish) {
       save_output(w, s, n, source, o);
       atomicSub(w.control + C_PENDING, 1);

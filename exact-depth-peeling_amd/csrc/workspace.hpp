@@ -26,7 +26,7 @@ struct Ws {
   int nv, nf;
   Caps cap;
   int limit, all_layers;
-  int share_regions, propagate_ranks;
+  int share_regions, propagate_ranks, merge_regions;
 
   // status / metrics / control
   int *status;             // [16]
@@ -107,6 +107,9 @@ struct Ws {
   double2 *starts, *ends;
   int *partners;
   int *parents;            // [regions] union-find
+  int *choices;            // [regions] merge proposal (smallest patch-edge index)
+  double2 *merge_xy;       // [MAXV][regions] compaction pool after merging
+  int *merge_n, *merge_src, *merge_layer;  // [regions]
   int *eslots;             // [ehash2_size] reversed-edge hash
   int ehash2_size;
 
@@ -117,8 +120,8 @@ struct Ws {
   int2 *tasks;              // [vertices]
   int *crowded;             // [vertices] edges for the per-lane-array noding path
   int *phash;               // [phash_size] boundary vertex dedupe (layer, x, y)
-  unsigned long long *cell_keys;  // [phash_size] (layer, cell) keys of the vertex grid
-  int *cell_heads;          // [phash_size] first vertex of each cell's list
+  int *cell_heads;          // [cell_capacity] first vertex of each (layer, cell) list
+  long cell_capacity;       // 4 * vertices: layers * G^2 never exceeds it
   int *cell_next;           // [vertices] next vertex in the same cell
   int phash_size;
   double *tri_uv;           // [triangles][3][2]
@@ -166,5 +169,7 @@ void enqueue_subdivide(const Ws &w, hipStream_t s, Timeline *t = nullptr);
 void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t = nullptr);
 void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t = nullptr);
 void enqueue_layered(const Ws &w, hipStream_t s, Timeline *t = nullptr);
+void enqueue_connect(const Ws &w, hipStream_t s);
+void enqueue_merge(const Ws &w, hipStream_t s);
 
 }  // namespace peel

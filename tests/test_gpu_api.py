@@ -467,16 +467,37 @@ def test_query_sort_bucket_replay():
 
 
 @pytest.mark.parametrize("name", ["holes", "partial", "closed_object", "coplanar_adjacent"])
-@pytest.mark.parametrize("propagate", [True, False])
-def test_component_switches(name, propagate):
-    """test_component_switches (test_components.py:64-78), merge off."""
+@pytest.mark.parametrize("propagate,merge", [(True, False), (False, True), (True, True), (False, False)])
+def test_component_switches(name, propagate, merge):
+    """test_components.py:64-78: rank propagation and convex coalescing leave
+    the labelled rank regions exact."""
     from hip_peel import EventPeeler
 
     arrays, labels, data, host = scene(name)
-    p = EventPeeler(*arrays, propagate_ranks=propagate)
+    p = EventPeeler(*arrays, propagate_ranks=propagate, merge_regions=merge)
     p.run()
     check_extraction(p, host)
     assert p.read_status()["edge_hash_misses"] == 0
+
+
+@pytest.mark.parametrize("propagate", [True, False])
+def test_merge_regions_coalesces(propagate):
+    """Convex coalescing (components.py:274-395) joins same-source, same-layer
+    patches across matched edges: fewer polygons, the same exact regions and
+    triangles, and the merged count reported."""
+    from hip_peel import EventPeeler, scenes
+    from hip_peel.io import upload
+
+    host = scenes.torus_knot(segments=32, sides=16)
+    plain = EventPeeler(*upload(*host[:4]), propagate_ranks=propagate)
+    plain.run()
+    merged = EventPeeler(*upload(*host[:4]), propagate_ranks=propagate, merge_regions=True)
+    merged.run()
+    a, b = plain.read_status(), merged.read_status()
+    assert b["merged_polygons"] > 0
+    assert b["polygons"] == a["polygons"] - b["merged_polygons"]
+    assert b["layers"] == a["layers"]
+    check_extraction(merged, host[:4], grid=32)
 
 
 def test_event_edge_capacity():
