@@ -10,7 +10,8 @@ and readback are outside the timed region (the reference's protocol,
 scripts/benchmark_events.py:77-93). The headline workload is the 65,536-face
 (2,3) torus-knot tube, timed with binary32-filtered ray predicates (`value`)
 and with plain binary64 predicates; the 717,740-face nested shells are timed
-both ways as well. --extras adds the 65,536-layer identical-footprint stack.
+both ways as well. --extras adds the 65,536-layer identical-footprint stack
+and 256 uneven icospheres peeled by one batched graph launch.
 
 Prints one JSON line. ``roofline`` prices the dominant kernel's algorithmic
 bytes per launch against its own event-timed duration; ``cpu_baseline`` times
@@ -38,7 +39,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="torus-knot-65536")
-    ap.add_argument("--extras", action="store_true", help="also time the identical-footprint stack")
+    ap.add_argument("--extras", action="store_true", help="also time the stack and the batched icospheres")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-large", action="store_true", help="skip the 717,740-face nested shells")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -116,6 +117,79 @@ def kernel_bytes(name, host, info, nverts):
         # samples in, hits out (depth 8 + source 4 + group 4)
         return 16 * P + 16 * info["ray_hits"]
     return None
+
+
+def run_batch(steps, warmup, count=256, seed=0):
+    """Batched uneven instances: ``count`` noisy icospheres (320 to 81,920
+    faces each), each under its own pinhole, peeled by one graph launch
+    (BatchPeeler); timed beside the members' own graphs launched back to back
+    on their own streams. Pools start from the estimate; members that overflow
+    it are rebuilt with the conservative capacities."""
+    import torch
+
+    from hip_peel import BatchPeeler, EventPeeler, scenes
+    from hip_peel.io import upload
+
+    meshes = scenes.icosphere_instances(count, seed=seed)
+    wall = time.perf_counter()
+    batch = BatchPeeler.from_meshes(meshes, filter_fp32=True)
+    setup_s = time.perf_counter() - wall
+    root = batch.stream  # the first member's stream: the batch graph forks and joins there
+
+    def timed(launch):
+        for _ in range(warmup):
+            launch()
+        torch.cuda.synchronize()
+        per = []
+        for _ in range(steps):
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            start.record(root)
+            launch()
+            end.record(root)
+            end.synchronize()
+            per.append(start.elapsed_time(end))
+        torch.cuda.synchronize()
+        return sorted(per)[len(per) // 2]
+
+    batch_ms = timed(batch.run)
+    infos = batch.read_status()
+    for p in batch.peelers:
+        p.capture()
+
+    def separate():
+        fork = torch.cuda.Event()
+        fork.record(root)
+        for p in batch.peelers:
+            p.stream.wait_event(fork)
+            p.run()
+        for p in batch.peelers:
+            root.wait_stream(p.stream)
+
+    separate_ms = timed(separate)
+    # pool memory of the estimate against the conservative capacities, per distinct size
+    conservative = {}
+    for m in meshes:
+        n = len(m[1])
+        if n not in conservative:
+            probe = EventPeeler(*upload(*m[:4]))
+            conservative[n] = probe.device_bytes
+            del probe
+    faces = sum(len(m[1]) for m in meshes)
+    return {
+        "instances": count,
+        "faces": faces,
+        "batch_graph_ms": round(batch_ms, 3),
+        "separate_graphs_ms": round(separate_ms, 3),
+        "batch_speedup": round(separate_ms / batch_ms, 3),
+        "faces_per_s": round(faces / (batch_ms * 1e-3)),
+        "layers_max": max(i["layers"] for i in infos),
+        "polygons": sum(i["polygons"] for i in infos),
+        "triangles": sum(i["triangles"] for i in infos),
+        "rebuilt_members": len(batch.rebuilt),
+        "pool_gib_estimated": round(sum(p.device_bytes for p in batch.peelers) / 2**30, 3),
+        "pool_gib_conservative": round(sum(conservative[len(m[1])] for m in meshes) / 2**30, 3),
+        "setup_s": round(setup_s, 2),
+    }
 
 
 def run_config(name, steps, warmup, filter_fp32=False, host=None):
@@ -270,6 +344,7 @@ def main():
                            "polygons": i["polygons"], "triangles": i["triangles"],
                            "stages_ms": {k: round(v, 3) for k, v in st.items() if k.startswith("k_")}}
             del p
+        extra["icospheres-256"] = run_batch(max(3, args.steps // 2), 1)
         result["extra"] = extra
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)

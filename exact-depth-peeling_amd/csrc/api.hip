@@ -23,6 +23,7 @@ struct PeelWorkspace {
   bool warmed = false;
   int device = 0;
   std::vector<void *> allocations;
+  int64_t bytes = 0;  // device memory held by the pools
 };
 
 static thread_local std::string g_error;
@@ -50,6 +51,7 @@ static int alloc(PeelWorkspace *ws, T *&ptr, long count) {
   e = hipMemset(p, 0, sizeof(T) * (size_t)count);
   if (e != hipSuccess) return fail(-(int)e, std::string("hipMemset: ") + hipGetErrorString(e));
   ws->allocations.push_back(p);
+  ws->bytes += (int64_t)(sizeof(T) * (size_t)count);
   ptr = (T *)p;
   return 0;
 }
@@ -425,6 +427,98 @@ int peel_get_buffers(PeelWorkspace *ws, PeelBuffers *b) {
   b->stream = (void *)ws->stream;
   b->stats = (int64_t *)w.stats;
   b->stats_rows = w.cap.layers;
+  b->device_bytes = ws->bytes;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of workspaces in one graph.
+struct PeelBatch {
+  std::vector<PeelWorkspace *> members;
+  std::vector<hipEvent_t> events;  // fork (one) and joins (one per member)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int peel_batch_destroy(PeelBatch *b) {
+  if (!b) return 0;
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  for (hipEvent_t e : b->events) hipEventDestroy(e);
+  delete b;
+  return 0;
+}
+
+int peel_batch_create(PeelWorkspace *const *list, int32_t count, PeelBatch **out) {
+  if (!list || !out || count < 1) return fail(1, "null argument or empty batch");
+  *out = nullptr;
+  for (int i = 0; i < count; ++i) {
+    if (!list[i]) return fail(1, "null workspace in batch");
+    for (int j = 0; j < i; ++j)
+      if (list[j] == list[i]) return fail(2, "a workspace appears twice in the batch");
+    if (list[i]->device != list[0]->device) return fail(2, "batch workspaces must share one device");
+  }
+  for (int i = 0; i < count; ++i) {
+    if (!list[i]->warmed) {
+      int r = peel_warmup(list[i]);
+      if (r) return r;
+    }
+  }
+  PeelBatch *b = new PeelBatch();
+  b->members.assign(list, list + count);
+  b->events.resize(count + 1, nullptr);
+  for (auto &e : b->events) {
+    hipError_t err = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (err != hipSuccess) {
+      peel_batch_destroy(b);
+      return fail(-(int)err, std::string("hipEventCreate: ") + hipGetErrorString(err));
+    }
+  }
+  hipStream_t root = list[0]->stream;
+  hipError_t err = hipStreamBeginCapture(root, hipStreamCaptureModeThreadLocal);
+  if (err != hipSuccess) {
+    peel_batch_destroy(b);
+    return fail(-(int)err, std::string("hipStreamBeginCapture: ") + hipGetErrorString(err));
+  }
+  // fork: every other member stream waits on the root, joining the capture
+  err = hipEventRecord(b->events[0], root);
+  for (int i = 1; i < count && err == hipSuccess; ++i) err = hipStreamWaitEvent(list[i]->stream, b->events[0], 0);
+  for (int i = 0; i < count && err == hipSuccess; ++i) {
+    enqueue_all(list[i]->w, list[i]->stream, nullptr);
+    err = hipGetLastError();
+  }
+  // join: the root waits on every member stream
+  for (int i = 1; i < count && err == hipSuccess; ++i) {
+    err = hipEventRecord(b->events[i], list[i]->stream);
+    if (err == hipSuccess) err = hipStreamWaitEvent(root, b->events[i], 0);
+  }
+  hipGraph_t g = nullptr;
+  hipError_t end = hipStreamEndCapture(root, &g);
+  if (err == hipSuccess) err = end;
+  if (err != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    peel_batch_destroy(b);
+    return fail(-(int)err, std::string("batch capture: ") + hipGetErrorString(err));
+  }
+  b->graph = g;
+  err = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  if (err != hipSuccess) {
+    peel_batch_destroy(b);
+    return fail(-(int)err, std::string("hipGraphInstantiate: ") + hipGetErrorString(err));
+  }
+  *out = b;
+  return 0;
+}
+
+int peel_batch_launch(PeelBatch *b) {
+  if (!b) return fail(1, "null batch");
+  HIP_TRY(hipGraphLaunch(b->exec, b->members[0]->stream));
+  return 0;
+}
+
+int peel_batch_synchronize(PeelBatch *b) {
+  if (!b) return fail(1, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->members[0]->stream));
   return 0;
 }
 

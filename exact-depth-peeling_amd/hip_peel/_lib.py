@@ -59,6 +59,7 @@ class PeelBuffers(ctypes.Structure):
         ("stream", ctypes.c_void_p),
         ("stats", ctypes.c_void_p),
         ("stats_rows", ctypes.c_int32),
+        ("device_bytes", ctypes.c_int64),
     ]
 
 
@@ -75,6 +76,10 @@ EXPORTS = (
     "peel_destroy",
     "peel_last_error",
     "peel_version",
+    "peel_batch_create",
+    "peel_batch_launch",
+    "peel_batch_synchronize",
+    "peel_batch_destroy",
 )
 
 _lib = None
@@ -102,6 +107,11 @@ def load():
     lib.peel_read_status.restype = ctypes.c_int
     lib.peel_profile.argtypes = [vp, vp, vp, ctypes.c_char_p, ctypes.c_int32]
     lib.peel_profile.restype = ctypes.c_int
+    lib.peel_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]
+    lib.peel_batch_create.restype = ctypes.c_int
+    for name in ("peel_batch_launch", "peel_batch_synchronize", "peel_batch_destroy"):
+        getattr(lib, name).argtypes = [vp]
+        getattr(lib, name).restype = ctypes.c_int
     lib.peel_last_error.restype = ctypes.c_char_p
     lib.peel_version.restype = ctypes.c_char_p
     _lib = lib

@@ -229,6 +229,43 @@ def nested_shells(count=13, r0=1.0, step=0.1, amplitude=0.02, seed=0):
     return vertices, faces, np.eye(4), settings
 
 
+def pinhole_look_at(eye, target=(0.0, 0.0, 0.0)):
+    """Affine world-to-camera matrix (4 x 4) of a pinhole at ``eye`` looking at
+    ``target``: camera z forward (depth > 0 in front), det +1."""
+    eye = np.asarray(eye, dtype=np.float64)
+    z = np.asarray(target, dtype=np.float64) - eye
+    z /= np.linalg.norm(z)
+    up = np.array([0.0, 0.0, 1.0]) if abs(z[2]) < 0.99 else np.array([0.0, 1.0, 0.0])
+    x = np.cross(up, z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    m = np.eye(4)
+    m[:3, :3] = np.stack([x, y, z])
+    m[:3, 3] = -m[:3, :3] @ eye
+    return m
+
+
+def icosphere_instances(count=256, seed=0, levels=(2, 6), amplitude=0.3, distance=3.0):
+    """Batched uneven instances: ``count`` icospheres subdivided ``levels``
+    times (20 * 4^L faces, L drawn uniformly by seed), radially displaced by
+    3-octave value noise of the given amplitude, each with its own normalized
+    pinhole (window [-1, 1]^2, near 0.1) at ``distance`` from the origin,
+    aimed at it from a random azimuth and elevation. Returns a list of
+    (vertices, faces, camera, settings)."""
+    rng = np.random.default_rng(seed)
+    settings = np.array([-1.0, -1.0, 1.0, 1.0, 1.0, 0.1])
+    out = []
+    for k in range(count):
+        level = int(rng.integers(levels[0], levels[1] + 1))
+        azimuth = rng.uniform(0.0, 2.0 * np.pi)
+        elevation = rng.uniform(-0.5 * np.pi, 0.5 * np.pi)
+        eye = distance * np.array(
+            [np.cos(elevation) * np.cos(azimuth), np.cos(elevation) * np.sin(azimuth), np.sin(elevation)])
+        vertices, faces = geodesic_sphere(1.0, 2**level, amplitude=amplitude, seed=seed * 100003 + k)
+        out.append((vertices, faces, pinhole_look_at(eye), settings.copy()))
+    return out
+
+
 def random_triangles(n=256, seed=0):
     rng = np.random.default_rng(seed)
     vertices = rng.random((3 * n, 3))

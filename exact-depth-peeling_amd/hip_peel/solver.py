@@ -169,6 +169,7 @@ class _Workspace:
         self.metrics = device_view(b.metrics, (b.metrics_count,), torch.int64, dev)
         self.canonical = device_view(b.canonical, (max(1, self.nv),), torch.int32, dev)
         self.stats = device_view(b.stats, (b.stats_rows, 3), torch.int64, dev)
+        self.device_bytes = int(b.device_bytes)
         self.stream = torch.cuda.ExternalStream(b.stream, device=dev)
         self.graph_captured = False
         self.warmed = False

@@ -86,6 +86,7 @@ typedef struct PeelBuffers {
   int64_t *stats;      /* layered route: [stats_rows][3] targets, candidate tests,
                           fully hidden targets per layer (solver.py:153) */
   int32_t stats_rows;
+  int64_t device_bytes; /* device memory held by the workspace's pools */
 } PeelBuffers;
 
 /* Allocate every pool and create the workspace stream
@@ -128,6 +129,20 @@ int peel_synchronize(PeelWorkspace *ws);
 
 /* Release every pool, the graph and the stream. */
 int peel_destroy(PeelWorkspace *ws);
+
+/* Batches: one graph that replays several workspaces' pipelines, each on its
+ * own stream, forked from and joined to the first workspace's stream, so
+ * independent meshes (each with its own camera) peel in one launch and may
+ * overlap on the GPU. Every workspace must stay alive, and must not be
+ * launched on its own, while the batch is in use; each keeps its own outputs
+ * and status (read them with peel_read_status after peel_batch_synchronize).
+ * No counterpart in the reference, which peels one mesh per graph
+ * (warp_peel/events.py:798-806); it serves callers with many small meshes. */
+typedef struct PeelBatch PeelBatch;
+int peel_batch_create(PeelWorkspace *const *workspaces, int32_t count, PeelBatch **out);
+int peel_batch_launch(PeelBatch *batch);
+int peel_batch_synchronize(PeelBatch *batch);
+int peel_batch_destroy(PeelBatch *batch);
 
 /* Message of the last failure on this thread ("" if none). */
 const char *peel_last_error(void);

@@ -234,13 +234,32 @@ def cpu_baseline(host, seconds=20.0, threads=1, workload=""):
         times.append(time.perf_counter() - t)
         runs += 1
     times.sort()
+    # fewer threads, a few runs each (the reference's CPU protocol times 1, 4
+    # and all cores, scripts/benchmark_event_cpu.py:41-62)
+    variants = {}
+    for t in sorted({1, 4} - {int(threads)}):
+        if t > threads:
+            continue
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            cpu_event_peel(*host[:4], threads=t)
+            ts.append(time.perf_counter() - t0)
+        variants[f"{t}_threads_ms"] = round(1e3 * sorted(ts)[1], 3)
+    cpu_max = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            cpu_max = f.read().strip()
+    except OSError:
+        pass
     return {
         "value": round(1e3 * times[len(times) // 2], 3),
         "unit": "ms",
         "cores": int(threads),
         "kind": "port",
         "sample": f"{runs} full peels of {workload} ({len(host[1])} faces), median; "
-        f"host nproc={os.cpu_count()}",
+        f"host nproc={os.cpu_count()}, cgroup cpu.max={cpu_max}",
+        "fewer_threads": variants,
         "stage_seconds": [round(x, 4) for x in result.seconds],
         "polygons": result.counts["polygons"],
         "layers": result.counts["layers"],
