@@ -278,6 +278,18 @@ def random_triangles(n=256, seed=0):
     return vertices, faces, camera, settings
 
 
+def random_triangles_resolved(n=256, seed=0, workers=1):
+    """``random_triangles`` with its surface intersections split exactly
+    (``resolve.resolve_arrays``), the input contract of the contour routes.
+    Returns (vertices, faces, camera, settings, parents); the window and camera
+    are those of the unresolved scene."""
+    from .resolve import resolve_arrays
+
+    vertices, faces, camera, settings = random_triangles(n, seed)
+    rv, rf, parents = resolve_arrays(vertices, faces, workers=workers)
+    return rv, rf, camera, settings, parents
+
+
 CONFIGS = {
     "torus-knot-65536": torus_knot,
     "stack-65536": stack,

@@ -7,8 +7,11 @@
 // that fragment's layer equal to the exact rank; no fragment of a source that
 // does not cover the point may contain it. Containment in a fragment (a convex
 // counter-clockwise polygon with binary64 vertices, normalised coordinates) is
-// decided exactly; points on a fragment boundary are skipped, like points
-// closer than 1e-11 to an edge in the reference audit (audit_gpu.py:78-80).
+// decided exactly; a point closer than 1e-11 to a fragment's boundary makes
+// that fragment's ownership ambiguous, as the reference audit skips such
+// points (audit_gpu.py:78-80): fragment vertices are binary64 constructions,
+// so a point within rounding distance of a source edge may fall on either
+// side of the fragment's rounded edge.
 #include "expansion.hpp"
 
 #include <algorithm>
@@ -31,6 +34,16 @@ int orient_sign(double ax, double ay, double bx, double by, double cx, double cy
   Exp bxax = two(bx, -ax), cyay = two(cy, -ay), byay = two(by, -ay), cxax = two(cx, -ax);
   return sub(mul(bxax, cyay), mul(byay, cxax)).sign();
 }
+
+// Distance from p to the segment ab (binary64; only compared with 1e-11).
+double segment_distance(double px, double py, double ax, double ay, double bx, double by) {
+  double dx = bx - ax, dy = by - ay, len2 = dx * dx + dy * dy;
+  double t = len2 > 0 ? ((px - ax) * dx + (py - ay) * dy) / len2 : 0.0;
+  t = std::min(1.0, std::max(0.0, t));
+  return std::hypot(px - (ax + t * dx), py - (ay + t * dy));
+}
+
+constexpr double BOUNDARY_DISTANCE = 1e-11;
 
 }  // namespace
 
@@ -86,7 +99,11 @@ void oracle_audit_fragments(const long *frag_offsets, const double *frag_xy, con
     for (int f : grid[gy * g + gx]) {
       int state = 1;
       long b = frag_offsets[f], n = frag_offsets[f + 1] - b;
-      for (long v = 0; v < n && state != -1; ++v) {
+      for (long v = 0; v < n && state == 1; ++v) {
+        const double *p = frag_xy + 2 * (b + v), *q = frag_xy + 2 * (b + (v + 1) % n);
+        if (segment_distance(qx, qy, p[0], p[1], q[0], q[1]) < BOUNDARY_DISTANCE) state = 0;
+      }
+      for (long v = 0; v < n && state == 1; ++v) {
         const double *p = frag_xy + 2 * (b + v), *q = frag_xy + 2 * (b + (v + 1) % n);
         int s = orient_sign(p[0], p[1], q[0], q[1], qx, qy);
         if (s < 0) state = -1;

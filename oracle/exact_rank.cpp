@@ -34,6 +34,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 using namespace oracle;
@@ -274,61 +275,75 @@ long oracle_exact_ranks(const double *vertices, int nv, const int *faces, int nf
   Scene s = make_scene(vertices, nv, faces, nf, camera, settings);
   double sx0 = s.w[0], sy0 = s.w[1];
   Exp wx = two(s.w[2], -s.w[0]), wy = two(s.w[3], -s.w[1]);
+  // Points are independent: each thread ranks a share of them into its own
+  // lists; the CSR is then written in point order.
+  std::vector<std::vector<std::pair<int, int>>> lists(nq);
+#pragma omp parallel
+  {
+    std::vector<Hit> hits;
+#pragma omp for schedule(dynamic, 64)
+    for (int i = 0; i < nq; ++i) {
+      double qx = points[2 * i], qy = points[2 * i + 1];
+      int pflag = 0;
+      hits.clear();
+      if (!(qx > 0 && qx < 1 && qy > 0 && qy < 1)) {
+        pflag |= FLAG_WINDOW;
+      } else {
+        Exp ux = add(Exp(sx0), scale(wx, qx));
+        Exp uy = add(Exp(sy0), scale(wy, qy));
+        int gx = std::min(s.g - 1, (int)(qx * s.g)), gy = std::min(s.g - 1, (int)(qy * s.g));
+        int cell = gy * s.g + gx;
+        for (int k = s.cell_start[cell]; k < s.cell_start[cell + 1]; ++k) {
+          int f = s.cell_items[k];
+          const Source &src = s.sources[f];
+          if (qx < src.lo[0] || qx > src.hi[0] || qy < src.lo[1] || qy > src.hi[1]) continue;
+          Hit h;
+          int c = cover(s, src, f, ux, uy, h);
+          if (c < 0) pflag |= FLAG_EDGE;
+          if (c > 0) hits.push_back(std::move(h));
+        }
+      }
+      // Exact front-to-back order; equal depths by source index.
+      std::vector<int> order(hits.size());
+      for (size_t k = 0; k < hits.size(); ++k) order[k] = (int)k;
+      std::vector<int> tie(hits.size(), 0);
+      std::sort(order.begin(), order.end(), [&](int a, int b) {
+        const Hit &ha = hits[a], &hb = hits[b];
+        double da = ha.approx, db = hb.approx;
+        double scale_ab = std::fabs(da) + std::fabs(db);
+        if (std::isfinite(da) && std::isfinite(db) && std::fabs(da - db) > 1e-6 * scale_ab + 1e-300)
+          return da < db;
+        int c = compare_depth(ha, hb);
+        if (c == 0) {
+          tie[a] = tie[b] = 1;
+          return ha.source < hb.source;
+        }
+        return c < 0;
+      });
+      // std::sort may not compare every adjacent pair; confirm ties explicitly.
+      for (size_t k = 0; k + 1 < order.size(); ++k)
+        if (compare_depth(hits[order[k]], hits[order[k + 1]]) == 0) tie[order[k]] = tie[order[k + 1]] = 1;
+      auto &out = lists[i];
+      out.reserve(order.size());
+      for (size_t k = 0; k < order.size(); ++k) {
+        int f = tie[order[k]] ? FLAG_TIE : 0;
+        pflag |= f;
+        out.emplace_back(hits[order[k]].source, f);
+      }
+      point_flags[i] = pflag;
+    }
+  }
   long total = 0;
-  std::vector<Hit> hits;
   for (int i = 0; i < nq; ++i) {
     offsets[i] = total;
-    double qx = points[2 * i], qy = points[2 * i + 1];
-    int pflag = 0;
-    hits.clear();
-    if (!(qx > 0 && qx < 1 && qy > 0 && qy < 1)) {
-      pflag |= FLAG_WINDOW;
-    } else {
-      Exp ux = add(Exp(sx0), scale(wx, qx));
-      Exp uy = add(Exp(sy0), scale(wy, qy));
-      int gx = std::min(s.g - 1, (int)(qx * s.g)), gy = std::min(s.g - 1, (int)(qy * s.g));
-      int cell = gy * s.g + gx;
-      for (int k = s.cell_start[cell]; k < s.cell_start[cell + 1]; ++k) {
-        int f = s.cell_items[k];
-        const Source &src = s.sources[f];
-        if (qx < src.lo[0] || qx > src.hi[0] || qy < src.lo[1] || qy > src.hi[1]) continue;
-        Hit h;
-        int c = cover(s, src, f, ux, uy, h);
-        if (c < 0) pflag |= FLAG_EDGE;
-        if (c > 0) hits.push_back(std::move(h));
-      }
-    }
-    // Exact front-to-back order; equal depths by source index.
-    std::vector<int> order(hits.size());
-    for (size_t k = 0; k < hits.size(); ++k) order[k] = (int)k;
-    std::vector<int> tie(hits.size(), 0);
-    std::sort(order.begin(), order.end(), [&](int a, int b) {
-      const Hit &ha = hits[a], &hb = hits[b];
-      double da = ha.approx, db = hb.approx;
-      double scale_ab = std::fabs(da) + std::fabs(db);
-      if (std::isfinite(da) && std::isfinite(db) && std::fabs(da - db) > 1e-6 * scale_ab + 1e-300)
-        return da < db;
-      int c = compare_depth(ha, hb);
-      if (c == 0) {
-        tie[a] = tie[b] = 1;
-        return ha.source < hb.source;
-      }
-      return c < 0;
-    });
-    // std::sort may not compare every adjacent pair; confirm ties explicitly.
-    for (size_t k = 0; k + 1 < order.size(); ++k)
-      if (compare_depth(hits[order[k]], hits[order[k + 1]]) == 0) tie[order[k]] = tie[order[k + 1]] = 1;
-    for (size_t k = 0; k < order.size(); ++k) {
-      int f = tie[order[k]] ? FLAG_TIE : 0;
-      pflag |= f;
+    for (size_t k = 0; k < lists[i].size(); ++k) {
       if (total < cap) {
-        sources[total] = hits[order[k]].source;
+        sources[total] = lists[i][k].first;
         ranks[total] = (int)k;
-        flags[total] = f;
+        flags[total] = lists[i][k].second;
       }
       ++total;
     }
-    point_flags[i] = pflag;
   }
   offsets[nq] = total;
   return total;

@@ -216,3 +216,35 @@ def test_audit_detects_planted_mistakes():
     assert dup["duplicate"] == 3
     spurious = oracle.audit(rings + [ring], [0, 1, 2, 7], [0, 1, 2, 3], pts, ranks)
     assert spurious["spurious"] == 3
+
+
+def test_audit_boundary_distance():
+    """A fragment whose edge passes within 1e-11 of a sample point is
+    ambiguous there (audit_gpu.py:78-80 skips such points), while a fragment
+    short of the point by 1e-9 is a reported gap. The rational fixture
+    random_04 places a grid point on a source edge in real arithmetic; its
+    binary64 coordinates sit ~1e-17 inside the exact source and outside the
+    fragment's rounded edge."""
+    V, F, C, S = stack_scene(1)
+    ring = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    ring = (ring - S[:2]) / (S[2:4] - S[:2])
+    pts = np.array([[0.3, 0.2]])
+    ranks = oracle.exact_ranks(V, F, C, S, pts)
+    assert ranks.at(0) == [0]
+    # move the hypotenuse (outward normal) past the point
+    normal = np.array([1.0, 1.0]) / np.sqrt(2.0)
+    edge = ring[1:]
+    d_point = float(np.dot(pts[0] - edge[0], normal))
+    # the point ends up outside the moved edge by 1e-12, then by 1e-9
+    for offset, missing in ((d_point - 1e-12, 0), (d_point - 1e-9, 1)):
+        moved = ring.copy()
+        moved[1:] += offset * normal
+        report = oracle.audit([moved], [0], [0], pts, ranks)
+        assert report["missing"] == missing, (offset, report)
+    from peel_checks import fixture_path, grid_points
+
+    host = oracle.load_scene(fixture_path("random_04"))[:4]
+    cpu = oracle.cpu_event_peel(*host, threads=1)
+    gp = grid_points(48)
+    report = oracle.audit(cpu.rings, cpu.sources, cpu.layers, gp, oracle.exact_ranks(*host, gp))
+    assert report["mismatches"] == 0 and report["pairs_on_fragment_boundary"] > 0
