@@ -123,8 +123,8 @@ def run_batch(steps, warmup, count=256, seed=0):
     """Batched uneven instances: ``count`` noisy icospheres (320 to 81,920
     faces each), each under its own pinhole, peeled by one graph launch
     (BatchPeeler); timed beside the members' own graphs launched back to back
-    on their own streams. Pools start from the estimate; members that overflow
-    it are rebuilt with the conservative capacities."""
+    on their own streams. Pools come from the learned pool sizes; members that
+    overflow them are rebuilt with the conservative capacities."""
     import torch
 
     from hip_peel import BatchPeeler, EventPeeler, scenes
@@ -166,7 +166,7 @@ def run_batch(steps, warmup, count=256, seed=0):
             root.wait_stream(p.stream)
 
     separate_ms = timed(separate)
-    # pool memory of the estimate against the conservative capacities, per distinct size
+    # pool memory of the learned sizes against the conservative capacities, per distinct size
     conservative = {}
     for m in meshes:
         n = len(m[1])
@@ -186,10 +186,34 @@ def run_batch(steps, warmup, count=256, seed=0):
         "polygons": sum(i["polygons"] for i in infos),
         "triangles": sum(i["triangles"] for i in infos),
         "rebuilt_members": len(batch.rebuilt),
-        "pool_gib_estimated": round(sum(p.device_bytes for p in batch.peelers) / 2**30, 3),
+        "pool_gib_learned": round(sum(p.device_bytes for p in batch.peelers) / 2**30, 3),
         "pool_gib_conservative": round(sum(conservative[len(m[1])] for m in meshes) / 2**30, 3),
         "setup_s": round(setup_s, 2),
     }
+
+
+def learned_pools(name):
+    """Pool memory of the learned sizes against the conservative capacities
+    for one workload, with the overflow-rerun outcome and equal outputs."""
+    import torch
+
+    from hip_peel import EventPeeler, pools, scenes
+    from hip_peel.io import upload
+
+    host = scenes.CONFIGS[name]()
+    arrays = upload(*host, device="cuda:0")
+    base = EventPeeler(*arrays)
+    base.run()
+    want = base.read_status()
+    peeler, rebuilt = pools.admitted_event_peeler(*arrays)
+    got = peeler.read_status()
+    assert all(got[k] == want[k] for k in ("polygons", "triangles", "layers"))
+    used = base.device_bytes if rebuilt else peeler.device_bytes
+    out = {"pool_gib_learned": round(used / 2**30, 3), "pool_gib_conservative": round(base.device_bytes / 2**30, 3),
+           "rebuilt": bool(rebuilt)}
+    del base, peeler
+    torch.cuda.synchronize()
+    return out
 
 
 def run_config(name, steps, warmup, filter_fp32=False, host=None):
@@ -325,6 +349,7 @@ def main():
             large[flag] = (i, t, st)
             del p
         (i64, t64, st64), (i32, t32, st32) = large[False], large[True]
+        learned = learned_pools(name)
         d32 = i32["filtered_predicates"] + i32["fp64_fallbacks"]
         result["fp32_vs_fp64"][name] = {
             "fp32_filtered_ms": round(t32["median_ms"], 3),
@@ -335,6 +360,7 @@ def main():
             "triangles": i32["triangles"],
             "fallback_fraction": round(i32["fp64_fallbacks"] / d32, 6) if d32 else None,
             "stages_ms": {k: round(v, 3) for k, v in st32.items() if k.startswith("k_")},
+            "learned_pools": learned,
         }
     if args.extras:
         extra = {}

@@ -80,6 +80,11 @@ EXPORTS = (
     "peel_batch_launch",
     "peel_batch_synchronize",
     "peel_batch_destroy",
+    "pool_mlp_param_count",
+    "pool_mlp_partial_count",
+    "pool_mlp_forward",
+    "pool_mlp_gradient",
+    "pool_mlp_adam",
 )
 
 _lib = None
@@ -112,6 +117,17 @@ def load():
     for name in ("peel_batch_launch", "peel_batch_synchronize", "peel_batch_destroy"):
         getattr(lib, name).argtypes = [vp]
         getattr(lib, name).restype = ctypes.c_int
+    i32, f32 = ctypes.c_int32, ctypes.c_float
+    lib.pool_mlp_param_count.argtypes = [i32, i32]
+    lib.pool_mlp_param_count.restype = i32
+    lib.pool_mlp_partial_count.argtypes = [i32, i32, i32]
+    lib.pool_mlp_partial_count.restype = i32
+    lib.pool_mlp_forward.argtypes = [vp, i32, i32, vp, i32, vp, vp]
+    lib.pool_mlp_forward.restype = ctypes.c_int
+    lib.pool_mlp_gradient.argtypes = [vp, i32, i32, vp, vp, i32, f32, vp, vp, vp, vp]
+    lib.pool_mlp_gradient.restype = ctypes.c_int
+    lib.pool_mlp_adam.argtypes = [vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, vp]
+    lib.pool_mlp_adam.restype = ctypes.c_int
     lib.peel_last_error.restype = ctypes.c_char_p
     lib.peel_version.restype = ctypes.c_char_p
     _lib = lib

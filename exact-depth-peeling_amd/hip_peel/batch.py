@@ -8,27 +8,17 @@ GPU. The reference peels one mesh per graph; this is an extension for callers
 with many small meshes (for example 256 icospheres of 320 to 81,920 faces,
 each seen by its own pinhole).
 
-``from_meshes`` sizes each workspace from an estimate and, when an estimate
-overflows, rebuilds that workspace with the reference's conservative
+``from_meshes`` sizes each workspace with the learned pool sizes
+(``pools.PoolModel``, one device inference for the whole batch) and, when a
+member overflows, rebuilds that workspace with the reference's conservative
 capacities and captures the batch again.
 """
 
 import ctypes
-from dataclasses import replace
 
 from . import _lib
 from .events import EventCapacities, EventPeeler
-
-CAPACITY_BITS = 4 | 8 | 16 | 32 | 128 | 1024 | 2048
-
-
-def estimated_capacities(n):
-    """Tighter pools for closed, star-shaped meshes seen from outside: a few
-    layers, few cuts. Overflow is detected and redone with
-    ``EventCapacities.for_mesh``."""
-    cap = EventCapacities.for_mesh(n)
-    m = max(16, n)
-    return replace(cap, frontier=6 * m, regions=3 * m, hits=8 * m, vertices=12 * m, triangles=8 * m)
+from .pools import CAPACITY_BITS
 
 
 class BatchPeeler:
@@ -47,29 +37,37 @@ class BatchPeeler:
         self.stream = self.peelers[0].stream
 
     @classmethod
-    def from_meshes(cls, meshes, device="cuda:0", estimate=True, **options):
+    def from_meshes(cls, meshes, device="cuda:0", pools="learned", model=None, **options):
         """Peelers for (vertices, faces, camera, settings) host arrays, each
-        uploaded to ``device``; with ``estimate`` the pools start from
-        ``estimated_capacities`` and an overflowing member is rebuilt with the
-        conservative defaults."""
+        uploaded to ``device``. ``pools="learned"`` sizes every workspace from
+        the pool network (``model`` or the trained default), runs the batch
+        once, and rebuilds each overflowing member with the conservative
+        capacities (``EventCapacities.for_mesh``); ``pools="conservative"``
+        uses those from the start. ``rebuilt`` lists the rebuilt members."""
+        from . import pools as pool_sizes
         from .io import upload
 
+        if pools not in ("learned", "conservative"):
+            raise ValueError("pools must be 'learned' or 'conservative'")
         arrays = [upload(*m[:4], device=device) for m in meshes]
-        peelers = []
-        for a in arrays:
-            n = a[1].shape[0]
-            cap = estimated_capacities(n) if estimate else EventCapacities.for_mesh(n)
-            peelers.append(EventPeeler(*a, capacities=cap, **options))
+        faces = [a[1].shape[0] for a in arrays]
+        if pools == "learned":
+            model = model or pool_sizes.load_default()
+            features = [pool_sizes.mesh_features(*m[:4]) for m in meshes]
+            caps = model.capacities_many(features, faces, device)
+        else:
+            caps = [EventCapacities.for_mesh(n) for n in faces]
+        peelers = [EventPeeler(*a, capacities=c, **options) for a, c in zip(arrays, caps)]
         batch = cls(peelers)
-        if estimate:
+        batch.rebuilt = []
+        if pools == "learned":
             batch.run()
             batch.synchronize()
             redo = [i for i, p in enumerate(batch.peelers) if int(p._read()[0][9]) & CAPACITY_BITS]
             if redo:
                 batch.close()
                 for i in redo:
-                    n = arrays[i][1].shape[0]
-                    peelers[i] = EventPeeler(*arrays[i], capacities=EventCapacities.for_mesh(n), **options)
+                    peelers[i] = EventPeeler(*arrays[i], capacities=EventCapacities.for_mesh(faces[i]), **options)
                 batch = cls(peelers)
             batch.rebuilt = redo
         return batch

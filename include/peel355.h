@@ -144,6 +144,31 @@ int peel_batch_launch(PeelBatch *batch);
 int peel_batch_synchronize(PeelBatch *batch);
 int peel_batch_destroy(PeelBatch *batch);
 
+/* Learned pool sizes. No counterpart in the reference, whose capacities are
+ * fixed multiples of the face count (warp_peel/events.py:19-33); it serves
+ * callers that admit many extractions into bounded memory. A multilayer
+ * perceptron (inputs <= 16 -> 64 tanh -> 64 tanh -> outputs <= 8, binary32,
+ * parameters flat [W1 b1 W2 b2 W3 b3], row-major) predicts per-pool usage
+ * from mesh statistics; an overflowing prediction is caught by the status
+ * bits and rerun with the conservative capacities. All pointers are device
+ * memory; `stream` is a hipStream_t (NULL = default stream); nothing here
+ * allocates or synchronises. Returns 0, or 1 for bad shapes/null pointers,
+ * 2 for bad scalars, -1 for a failed launch. */
+int32_t pool_mlp_param_count(int32_t inputs, int32_t outputs);
+/* Floats of the gradient workspace `partial` for a batch. */
+int32_t pool_mlp_partial_count(int32_t inputs, int32_t outputs, int32_t batch);
+/* y[batch][outputs] = network(x[batch][inputs]). */
+int pool_mlp_forward(const float *params, int32_t inputs, int32_t outputs, const float *x, int32_t batch,
+                     float *y, void *stream);
+/* grad[params] and *loss of the asymmetric squared error
+ * 1/(N O) sum w(r) r^2, r = y - target, w = under_weight where r < 0, else 1
+ * (loss may be NULL); deterministic (fixed summation order). */
+int pool_mlp_gradient(const float *params, int32_t inputs, int32_t outputs, const float *x, const float *target,
+                      int32_t batch, float under_weight, float *partial, float *grad, float *loss, void *stream);
+/* One Adam step; bias1 = 1 - beta1^t and bias2 = 1 - beta2^t. */
+int pool_mlp_adam(float *params, const float *grad, float *m, float *v, int32_t count, float lr, float beta1,
+                  float beta2, float eps, float bias1, float bias2, void *stream);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *peel_last_error(void);
 

@@ -12,7 +12,7 @@ from conftest import ROOT
 
 def declared():
     text = (ROOT / "include" / "peel355.h").read_text()
-    return sorted(set(re.findall(r"\b(peel_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:peel|pool_mlp)_[a-z_]+)\s*\(", text)))
 
 
 def test_header_declares_the_reference_lifecycle():

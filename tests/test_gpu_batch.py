@@ -24,7 +24,7 @@ def test_batch_matches_single_peels():
 
     print("[batch] 6 icospheres", flush=True)
     ms = meshes()
-    batch = BatchPeeler.from_meshes(ms, estimate=False)
+    batch = BatchPeeler.from_meshes(ms, pools="conservative")
     batch.run()
     infos = batch.read_status()
     for k, (m, p, info) in enumerate(zip(ms, batch.peelers, infos)):
@@ -41,12 +41,12 @@ def test_batch_matches_single_peels():
         check_extraction(p, m[:4], grid=32)
 
 
-def test_batch_replay_and_estimates():
-    """Estimated pools with a conservative rebuild on overflow, and replays
+def test_batch_replay_and_learned_pools():
+    """Learned pools with a conservative rebuild on overflow, and replays
     that reset every member."""
     from hip_peel import BatchPeeler
 
-    print("[batch] estimates and replay", flush=True)
+    print("[batch] learned pools and replay", flush=True)
     ms = meshes(count=8, levels=(2, 4), seed=5)
     batch = BatchPeeler.from_meshes(ms)
     batch.run()
@@ -58,20 +58,19 @@ def test_batch_replay_and_estimates():
 
 
 def test_batch_overflow_rebuilds_member():
-    """A member whose estimated pools overflow is rebuilt with the reference's
-    conservative capacities (the rerun-on-overflow protocol)."""
-    from dataclasses import replace
+    """A member whose learned pools overflow is rebuilt with the reference's
+    conservative capacities (the rerun-on-overflow protocol): a model whose
+    margins shrink every prediction far below use."""
+    import numpy as np
 
-    from hip_peel import BatchPeeler, batch as batch_module
+    from hip_peel import BatchPeeler, pools
 
     print("[batch] overflow rebuild", flush=True)
     ms = meshes(count=3, levels=(2, 2), seed=7)
-    original = batch_module.estimated_capacities
-    try:
-        batch_module.estimated_capacities = lambda n: replace(original(n), triangles=1)
-        b = BatchPeeler.from_meshes(ms)
-    finally:
-        batch_module.estimated_capacities = original
+    model = pools.load_default()
+    tight = pools.PoolModel(model.params, model.x_mean, model.x_std, model.y_mean, model.y_std,
+                            np.full(len(model.margin), -3.0))
+    b = BatchPeeler.from_meshes(ms, model=tight)
     assert b.rebuilt == [0, 1, 2]
     b.run()
     for m, p in zip(ms, b.peelers):
