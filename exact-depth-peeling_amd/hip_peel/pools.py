@@ -44,7 +44,11 @@ FEATURES = (
     "clipped_fraction",
     "perspective",
     "back_fraction",
+    "covered_fraction",
+    "log_depth_mean",
+    "log_depth_max",
 )
+DEPTH_GRID = 64
 # pool -> status key of what the pool holds after an extraction
 TARGETS = (
     ("frontier", "frontier_slots"),
@@ -123,7 +127,31 @@ def mesh_features(vertices, faces, camera, settings):
     out[8] = np.log1p(slot_len[contour & slot_ok].sum())
     out[9] = 1.0 - inside.mean()
     out[11] = float((signed < 0).mean())
+    out[12:15] = depth_complexity(tri[ok], area[ok])
     return out
+
+
+def depth_complexity(tri, area, grid=DEPTH_GRID):
+    """(covered fraction, log mean depth, log max depth) of projected faces:
+    each face's area is spread evenly over the window cells its bounding box
+    touches (a weighted difference array, then prefix sums), and a cell's
+    depth is the area it received over its own area."""
+    if len(tri) == 0:
+        return 0.0, 0.0, 0.0
+    lo = np.clip(np.floor(np.clip(tri.min(axis=1), 0.0, 1.0) * grid), 0, grid - 1).astype(np.int64)
+    hi = np.clip(np.floor(np.clip(tri.max(axis=1), 0.0, 1.0) * grid), 0, grid - 1).astype(np.int64)
+    cells = (hi[:, 0] - lo[:, 0] + 1) * (hi[:, 1] - lo[:, 1] + 1)
+    w = area / cells
+    diff = np.zeros((grid + 1, grid + 1))
+    np.add.at(diff, (lo[:, 1], lo[:, 0]), w)
+    np.add.at(diff, (lo[:, 1], hi[:, 0] + 1), -w)
+    np.add.at(diff, (hi[:, 1] + 1, lo[:, 0]), -w)
+    np.add.at(diff, (hi[:, 1] + 1, hi[:, 0] + 1), w)
+    depth = diff.cumsum(axis=0).cumsum(axis=1)[:grid, :grid] * grid * grid
+    covered = depth > 1e-6
+    if not covered.any():
+        return 0.0, 0.0, 0.0
+    return float(covered.mean()), float(np.log1p(depth[covered].mean())), float(np.log1p(depth.max()))
 
 
 def usage_of(info):

@@ -122,3 +122,22 @@ def test_generated_meshes_cover_every_kind():
 
     kinds = [k for k, _ in pools.generated_meshes(40, seed=2)]
     assert set(kinds) == {"icosphere", "knot", "stack", "soup"}
+
+
+def test_depth_complexity_closed_form():
+    """Two stacked squares exactly filling the window: every cell covered at
+    depth 2; one triangle over half the window's cells touches all cells of
+    its bounding box at depth 1/2."""
+    from hip_peel import pools
+
+    names = pools.FEATURES
+    v = np.array([[0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=float)
+    f = np.array([[0, 1, 2], [0, 2, 3]])
+    v2 = np.concatenate([v, v + [0, 0, 1]])
+    f2 = np.concatenate([f, f + 4])
+    x = pools.mesh_features(v2, f2, np.eye(4), np.array([0.0, 0.0, 1.0, 1.0, 0.0, 0.01]))
+    assert x[names.index("covered_fraction")] == pytest.approx(1.0)
+    assert x[names.index("log_depth_mean")] == pytest.approx(np.log1p(2.0))
+    assert x[names.index("log_depth_max")] == pytest.approx(np.log1p(2.0))
+    x = pools.mesh_features(v, f[:1], np.eye(4), np.array([0.0, 0.0, 1.0, 1.0, 0.0, 0.01]))
+    assert x[names.index("log_depth_mean")] == pytest.approx(np.log1p(0.5))
