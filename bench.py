@@ -13,6 +13,10 @@ and with plain binary64 predicates; the 717,740-face nested shells are timed
 both ways as well. --extras adds the 65,536-layer identical-footprint stack
 and 256 uneven icospheres peeled by one batched graph launch.
 
+Before timing, each workload's launch tuning is autotuned (hip_peel.tune;
+outputs do not depend on it; --no-tune keeps the defaults), as a user tunes a
+workload once before replaying it.
+
 Prints one JSON line. ``roofline`` prices the dominant kernel's algorithmic
 bytes per launch against its own event-timed duration; ``cpu_baseline`` times
 the CPU FP64 restatement of the same route (oracle) on a bounded sample.
@@ -42,6 +46,7 @@ def parse():
     ap.add_argument("--extras", action="store_true", help="also time the stack and the batched icospheres")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-large", action="store_true", help="skip the 717,740-face nested shells")
+    ap.add_argument("--no-tune", action="store_true", help="time the default launch tuning only")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument(
         "--copy-calibration",
@@ -216,8 +221,18 @@ def learned_pools(name):
     return out
 
 
-def run_config(name, steps, warmup, filter_fp32=False, host=None):
+def run_config(name, steps, warmup, filter_fp32=False, host=None, tune=False):
+    """Time graph replays of one workload; with ``tune`` the launch tuning is
+    autotuned first (outputs do not depend on it) and the report kept on
+    ``peeler.tune_report``."""
     peeler, host = make_peeler(name, filter_fp32, host)
+    peeler.tune_report = None
+    if tune:
+        from hip_peel import tune as tuner
+
+        report = tuner.autotune(peeler, replays=5, rounds=2)
+        peeler.tune_report = {"best": report["best"], "default_ms": round(report["default_ms"], 4),
+                              "tuned_ms": round(report["best_ms"], 4), "trials": len(report["trials"])}
     timing = time_replays(peeler, steps, warmup)
     info = peeler.read_status()
     stages = peeler.profile()
@@ -276,10 +291,12 @@ def main():
         print(json.dumps({"copy_calibration_bytes": 8 * n, "launches": 3}))
         return
     # plain binary64 predicates first, then the binary32-filtered run (`value`)
-    p64, host, info64, stages64, timing64, _, _ = run_config(args.config, args.steps, args.warmup)
+    tune = not args.no_tune
+    p64, host, info64, stages64, timing64, _, _ = run_config(args.config, args.steps, args.warmup, tune=tune)
+    tune64 = p64.tune_report
     del p64
     peeler, host, info, stages, timing, total, nverts = run_config(args.config, args.steps, args.warmup,
-                                                                   filter_fp32=True, host=host)
+                                                                   filter_fp32=True, host=host, tune=tune)
     # the patch and triangle sets are deterministic (ray counters depend on
     # which patch represents a component, so they vary between runs)
     for key in ("polygons", "triangles", "layers"):
@@ -328,6 +345,8 @@ def main():
             args.config: {
                 "fp32_filtered_ms": round(timing["median_ms"], 4),
                 "fp64_ms": round(timing64["median_ms"], 4),
+                "tuning_fp32": peeler.tune_report,
+                "tuning_fp64": tune64,
                 "speedup": round(timing64["median_ms"] / timing["median_ms"], 3),
                 "filtered_predicates": info["filtered_predicates"],
                 "fp64_fallbacks": info["fp64_fallbacks"],
@@ -345,10 +364,10 @@ def main():
         name = "nested-shells-717740"
         large = {}
         for flag in (False, True):
-            p, h, i, st, t, tot, nv = run_config(name, max(3, args.steps // 2), 1, filter_fp32=flag)
-            large[flag] = (i, t, st)
+            p, h, i, st, t, tot, nv = run_config(name, max(3, args.steps // 2), 1, filter_fp32=flag, tune=tune)
+            large[flag] = (i, t, st, p.tune_report)
             del p
-        (i64, t64, st64), (i32, t32, st32) = large[False], large[True]
+        (i64, t64, st64, tr64), (i32, t32, st32, tr32) = large[False], large[True]
         learned = learned_pools(name)
         d32 = i32["filtered_predicates"] + i32["fp64_fallbacks"]
         result["fp32_vs_fp64"][name] = {
@@ -361,6 +380,8 @@ def main():
             "fallback_fraction": round(i32["fp64_fallbacks"] / d32, 6) if d32 else None,
             "stages_ms": {k: round(v, 3) for k, v in st32.items() if k.startswith("k_")},
             "learned_pools": learned,
+            "tuning_fp32": tr32,
+            "tuning_fp64": tr64,
         }
     if args.extras:
         extra = {}

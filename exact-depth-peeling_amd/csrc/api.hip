@@ -24,6 +24,8 @@ struct PeelWorkspace {
   int device = 0;
   std::vector<void *> allocations;
   int64_t bytes = 0;  // device memory held by the pools
+  int cus = 1;        // compute units of the device
+  int max_per_cu = 1; // resident k_subdivide blocks per CU (occupancy)
 };
 
 static thread_local std::string g_error;
@@ -274,13 +276,18 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device));
   int per_cu = subdivide_blocks_per_cu();
+  ws->cus = std::max(1, cus);
+  ws->max_per_cu = std::max(1, per_cu);
   // One 256-lane block per CU: enough lanes to hide traversal latency, few
   // enough idle pollers that the queue words do not become an L2 hot spot.
-  w.persistent_blocks = std::max(1, cus * std::max(1, std::min(per_cu, 1)));
+  w.persistent_blocks = ws->cus;
   if (const char *env = std::getenv("PEEL355_PERSISTENT_BLOCKS")) {
     int v = std::atoi(env);
-    if (v > 0) w.persistent_blocks = std::min(v, cus * std::max(1, per_cu));
+    if (v > 0) w.persistent_blocks = std::min(v, ws->cus * ws->max_per_cu);
   }
+  w.ray_lanes_target = 524288;
+  w.heavy_lanes = 65536;
+  w.long_edge_cells = 48;
   hipError_t e = hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     peel_destroy(ws);
@@ -378,6 +385,41 @@ int peel_capture(PeelWorkspace *ws) {
   if (e != hipSuccess) return fail(-(int)e, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   ws->graph = g;
   HIP_TRY(hipGraphInstantiate(&ws->exec, ws->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+int peel_get_tuning(PeelWorkspace *ws, PeelTuning *t) {
+  if (!ws || !t) return fail(1, "null argument");
+  const Ws &w = ws->w;
+  t->persistent_blocks_per_cu = std::max(1, w.persistent_blocks / ws->cus);
+  t->ray_lanes_target = w.ray_lanes_target;
+  t->heavy_lanes = w.heavy_lanes;
+  t->long_edge_cells = w.long_edge_cells;
+  t->max_blocks_per_cu = ws->max_per_cu;
+  return 0;
+}
+
+int peel_set_tuning(PeelWorkspace *ws, const PeelTuning *t) {
+  if (!ws || !t) return fail(1, "null argument");
+  if (t->persistent_blocks_per_cu < 1 || t->persistent_blocks_per_cu > ws->max_per_cu)
+    return fail(2, "persistent_blocks_per_cu must be in [1, " + std::to_string(ws->max_per_cu) + "]");
+  if (t->ray_lanes_target < 64) return fail(2, "ray_lanes_target must be at least 64");
+  if (t->heavy_lanes < 1024) return fail(2, "heavy_lanes must be at least 1024");
+  if (t->long_edge_cells < 1) return fail(2, "long_edge_cells must be positive");
+  Ws &w = ws->w;
+  w.persistent_blocks = ws->cus * t->persistent_blocks_per_cu;
+  w.ray_lanes_target = t->ray_lanes_target;
+  w.heavy_lanes = t->heavy_lanes;
+  w.long_edge_cells = t->long_edge_cells;
+  // the captured graph holds the old launch shapes and kernel arguments
+  if (ws->exec) {
+    hipGraphExecDestroy(ws->exec);
+    ws->exec = nullptr;
+  }
+  if (ws->graph) {
+    hipGraphDestroy(ws->graph);
+    ws->graph = nullptr;
+  }
   return 0;
 }
 

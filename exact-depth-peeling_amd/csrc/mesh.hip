@@ -239,7 +239,6 @@ struct EdgeQuery {
 };
 
 constexpr int REG_PARAMS = 8;
-constexpr int LONG_EDGE_CELLS = 48;  // grid columns + rows beyond which an edge is walked by a wave
 
 // Sorting network on 8 registers (Batcher odd-even merge), static indices only.
 __device__ __forceinline__ void cswap(double &x, double &y) {
@@ -344,7 +343,7 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
   if (valid) valid = edge_setup(w, j, q, center, i, layer);
   if (valid && q.length2 > 0.0) {
     const int G = load_i(w.control + C_GRID);
-    if (box_cells(q.a, q.b, G) > LONG_EDGE_CELLS) {
+    if (box_cells(q.a, q.b, G) > w.long_edge_cells) {
       crowded = true;  // long edges go straight to the wave-parallel path
     } else {
       walk_segment(w, q.a, q.b, layer, G, [&](int k) {

@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void k_certify_heavy(Ws w, int workers) {
   Tree tree = w.source_tree.tree();
   int jobs = ok ? load_i(w.control + C_BEAM_JOBS) : 0;
   int parts = min(8192, tree.leaves);
-  while (parts > 16 && (long long)parts * jobs > 65536) parts /= 2;
+  while (parts > 16 && (long long)parts * jobs > w.heavy_lanes) parts /= 2;
   int width = tree.leaves / parts;
   long total = (long)jobs * parts;
   long long visits = 0;
@@ -397,10 +397,12 @@ __device__ __forceinline__ void ray_box(double2 x, float4 &lo, float4 &hi) {
 // it passes 32 candidates (event_kernels.py:222-245): that counter keeps its
 // meaning.
 constexpr int HANDOFF = 32;
-constexpr long RAY_LANES_TARGET = 524288;  // lanes in flight that fill 256 CUs
 
-__device__ __forceinline__ int ray_lanes(const Tree &tree, int rays) {
-  if ((long)64 * rays > RAY_LANES_TARGET) return 1;
+// Lanes per ray: a whole wave (one leaf subtree per lane) while 64 lanes per
+// ray stay within `target` lanes in flight (524,288 by default: enough to fill
+// 256 CUs), else one.
+__device__ __forceinline__ int ray_lanes(const Tree &tree, int rays, int target) {
+  if ((long)64 * rays > (long)target) return 1;
   return tree.leaves < 64 ? tree.leaves : 64;
 }
 
@@ -438,7 +440,7 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
   const int n = load_i(w.control + C_LEADERS);
   const int lane = threadIdx.x & 63;
   const Tree tree = w.source_tree.tree();
-  const int K = ray_lanes(tree, n);
+  const int K = ray_lanes(tree, n, w.ray_lanes_target);
   const int sub = lane & (K - 1);
   const int per_wave = 64 / K;
   const long waves = (gridDim.x * blockDim.x) >> 6;
@@ -507,11 +509,11 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   int jobs = ok ? load_i(w.control + C_HEAVY) : 0;
   if (jobs > w.cap.regions) jobs = w.cap.regions;
   // leaves per lane subtree, with the lanes per ray that k_ray_hits chose
-  const int W = tree.leaves / ray_lanes(tree, load_i(w.control + C_LEADERS));
+  const int W = tree.leaves / ray_lanes(tree, load_i(w.control + C_LEADERS), w.ray_lanes_target);
   // a job split into more parts re-enters that many subtrees from their
-  // roots: split only while the split jobs still fit 65,536 lanes
+  // roots: split only while the split jobs still fit the heavy lane budget
   int parts = W;
-  while (parts > 1 && (long long)parts * jobs > 65536) parts /= 2;
+  while (parts > 1 && (long long)parts * jobs > w.heavy_lanes) parts /= 2;
   const int width = W / parts;
   long total = (long)jobs * parts;
   long long visits = 0;

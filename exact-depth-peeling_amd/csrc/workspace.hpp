@@ -147,6 +147,12 @@ struct Ws {
   SortScratch sort;
   ScanScratch scan;
   int persistent_blocks;    // grid of the persistent kernels
+  // launch tuning (PeelTuning): lanes in flight before rays drop from one
+  // subtree per lane to one lane per ray, subtree-job lanes of heavy rays and
+  // beams, grid cells beyond which an edge is noded by a whole wave
+  int ray_lanes_target;
+  int heavy_lanes;
+  int long_edge_cells;
 };
 
 // Optional host-side timeline: hipEvents recorded between stages when the

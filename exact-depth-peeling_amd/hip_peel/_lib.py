@@ -63,6 +63,16 @@ class PeelBuffers(ctypes.Structure):
     ]
 
 
+class PeelTuning(ctypes.Structure):
+    _fields_ = [
+        ("persistent_blocks_per_cu", ctypes.c_int32),
+        ("ray_lanes_target", ctypes.c_int32),
+        ("heavy_lanes", ctypes.c_int32),
+        ("long_edge_cells", ctypes.c_int32),
+        ("max_blocks_per_cu", ctypes.c_int32),
+    ]
+
+
 EXPORTS = (
     "peel_create",
     "peel_warmup",
@@ -76,6 +86,8 @@ EXPORTS = (
     "peel_destroy",
     "peel_last_error",
     "peel_version",
+    "peel_get_tuning",
+    "peel_set_tuning",
     "peel_batch_create",
     "peel_batch_launch",
     "peel_batch_synchronize",
@@ -112,6 +124,10 @@ def load():
     lib.peel_read_status.restype = ctypes.c_int
     lib.peel_profile.argtypes = [vp, vp, vp, ctypes.c_char_p, ctypes.c_int32]
     lib.peel_profile.restype = ctypes.c_int
+    lib.peel_get_tuning.argtypes = [vp, ctypes.POINTER(PeelTuning)]
+    lib.peel_get_tuning.restype = ctypes.c_int
+    lib.peel_set_tuning.argtypes = [vp, ctypes.POINTER(PeelTuning)]
+    lib.peel_set_tuning.restype = ctypes.c_int
     lib.peel_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]
     lib.peel_batch_create.restype = ctypes.c_int
     for name in ("peel_batch_launch", "peel_batch_synchronize", "peel_batch_destroy"):

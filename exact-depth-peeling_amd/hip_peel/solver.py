@@ -132,6 +132,7 @@ class _Workspace:
         self.vertices, self.faces, self.camera, self.settings = vertices, faces, camera, settings
         self.n = faces.shape[0]
         self.nv = vertices.shape[0]
+        self.flags = flags
         desc = _lib.PeelDesc(
             vertices.data_ptr() if self.nv else None,
             self.nv,
@@ -235,6 +236,28 @@ class _Workspace:
         _lib.check(self._lib.peel_profile(self._handle, ctypes.byref(count), ms, names, 4096))
         keys = names.value.decode().split(",")
         return dict(zip(keys, [float(ms[i]) for i in range(count.value)]))
+
+    @property
+    def tuning(self):
+        """Launch tuning (``PeelTuning`` fields as a dict, including the
+        read-only ``max_blocks_per_cu``)."""
+        t = _lib.PeelTuning()
+        _lib.check(self._lib.peel_get_tuning(self._handle, ctypes.byref(t)))
+        return {name: int(getattr(t, name)) for name, _ in t._fields_}
+
+    def set_tuning(self, **values):
+        """Change launch tuning; the graph is captured again at the next run.
+        Outputs do not depend on it."""
+        if torch.cuda.is_current_stream_capturing():
+            raise RuntimeError("Tuning cannot change during a capture")
+        current = self.tuning
+        unknown = set(values) - set(current) | ({"max_blocks_per_cu"} & set(values))
+        if unknown:
+            raise ValueError(f"Unknown or read-only tuning fields: {sorted(unknown)}")
+        current.update(values)
+        t = _lib.PeelTuning(**current)
+        _lib.check(self._lib.peel_set_tuning(self._handle, ctypes.byref(t)))
+        self.graph_captured = False
 
     def synchronize(self):
         _lib.check(self._lib.peel_synchronize(self._handle))

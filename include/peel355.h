@@ -124,6 +124,23 @@ int peel_read_status(PeelWorkspace *ws, int32_t *status, int64_t *metrics);
  * (replaces the per-stage diagnostics of scripts/profile_events.py:36-55). */
 int peel_profile(PeelWorkspace *ws, int32_t *count, float *ms, char *names, int32_t names_len);
 
+/* Launch tuning of one workspace. No counterpart in the reference, whose
+ * launch shapes are fixed (its north-star autotuner picks them here); it
+ * serves callers that tune a workload once and replay it many times
+ * (hip_peel.tune). Outputs do not depend on these values, only the time. */
+typedef struct PeelTuning {
+  int32_t persistent_blocks_per_cu; /* k_subdivide work-queue blocks per CU, 1..max_blocks_per_cu */
+  int32_t ray_lanes_target;         /* lanes in flight allowed at 64 lanes per ray (else 1 lane per ray), >= 64 */
+  int32_t heavy_lanes;              /* lane budget of the heavy-ray and beam subtree jobs, >= 1024 */
+  int32_t long_edge_cells;          /* grid cells beyond which a polygon edge is noded by a wave, >= 1 */
+  int32_t max_blocks_per_cu;        /* read only: resident k_subdivide blocks per CU */
+} PeelTuning;
+int peel_get_tuning(PeelWorkspace *ws, PeelTuning *tuning);
+/* Validates every field (2 on a bad value) and drops the captured graph: the
+ * next peel_launch captures again. A batch holding the workspace must be
+ * created again. */
+int peel_set_tuning(PeelWorkspace *ws, const PeelTuning *tuning);
+
 /* Synchronise the workspace stream. */
 int peel_synchronize(PeelWorkspace *ws);
 
