@@ -88,16 +88,31 @@ __device__ void sample_one(const Ws &w, int i) {
 
 // Exact sample groups: the first patch to claim a sample's slot leads it. The
 // samples compared here were written by k_samples (a previous kernel), so the
-// probe needs no fences.
-__device__ void sample_insert_one(const Ws &w, int i);
+// probe needs no fences. Only component representatives claim: every other
+// patch carries its representative's sample bit for bit, and k_groups finds
+// the claimed slot by probing. A representative whose left neighbour in the
+// wave is a representative with the same sample leaves the claim to it, so a
+// run of equal samples (the identical-footprint stack) issues one compare-and-
+// swap per wave instead of one per lane on a single address.
+__device__ void sample_insert_one(const Ws &w, int i, double2 x);
 
 __global__ void k_sample_insert(Ws w) {
   if (w.status[ST_ERR] != 0) return;
-  ITEM_LOOP(i, npolys(w)) sample_insert_one(w, i);
+  const int n = npolys(w);
+  const int lane = threadIdx.x & 63;
+  WAVE_LOOP(base, n) {
+    const int i = base + threadIdx.x;
+    const bool live = i < n;
+    const double2 x = live ? w.samples[i] : make_double2(0.0, 0.0);
+    const int claim = live && w.reps[i] == i;
+    const double left_x = __shfl_up(x.x, 1, 64), left_y = __shfl_up(x.y, 1, 64);
+    const int left_claim = __shfl_up(claim, 1, 64);
+    const bool repeat = lane > 0 && left_claim && left_x == x.x && left_y == x.y;
+    if (claim && !repeat) sample_insert_one(w, i, x);
+  }
 }
 
-__device__ void sample_insert_one(const Ws &w, int i) {
-  double2 x = w.samples[i];
+__device__ void sample_insert_one(const Ws &w, int i, double2 x) {
   int size = qhash_live(w), mask = size - 1;
   int slot = (int)(point_hash(x) & (uint64_t)mask);
   for (int probe = 0; probe < size; ++probe) {
