@@ -267,9 +267,11 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   sort_cap = std::max<long>(sort_cap, w.contour_tree.size);
   sort_cap = std::max<long>(sort_cap, R);
   w.sort.max_blocks = (int)((sort_cap + RS_TILE - 1) / RS_TILE);
-  ALLOC(w.sort.hist, 256L * w.sort.max_blocks);
-  ALLOC(w.sort.flags, RS_MAX_PASSES + 1);
-  ALLOC(w.sort.totals, 256);
+  ALLOC(w.sort.counts, RS_MAX_SWEEPS * 256);
+  ALLOC(w.sort.tickets, RS_MAX_SWEEPS);
+  ALLOC(w.sort.tiles, (long)RS_MAX_SWEEPS * w.sort.max_blocks * 256);
+  ALLOC(w.sort.flags, RS_MAX_SWEEPS);
+  w.sort.status = w.status;
   long scan_cap = std::max<long>(std::max<long>(R, BV), w.nseg);
   w.scan.max_blocks = (int)((scan_cap + 4095) / 4096) + 1;
   ALLOC(w.scan.block_sums, w.scan.max_blocks);

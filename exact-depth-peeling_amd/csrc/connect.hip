@@ -143,7 +143,9 @@ __device__ void match_one(const Ws &w, int i, int n) {
       int slot = (int)(key & (uint64_t)mask);
       missed = true;
       for (int attempt = 0; attempt < 64; ++attempt) {
-        int other = atomicCAS(&w.eslots[slot], -1, i);
+        // read first: only an empty-looking slot is claimed with an atomic
+        int other = load_i(&w.eslots[slot]);
+        if (other < 0) other = atomicCAS(&w.eslots[slot], -1, i);
         if (other < 0) {
           missed = false;
           break;
@@ -172,7 +174,8 @@ __device__ int find_root(int *parents, int i) {
     int parent = load_i(parents + node);
     if (parent == node) break;
     int grand = load_i(parents + parent);
-    atomicMin(parents + node, grand);
+    // path halving; a node already pointing at its root needs no atomic
+    if (grand != parent) atomicMin(parents + node, grand);
     node = grand;
   }
   return node;

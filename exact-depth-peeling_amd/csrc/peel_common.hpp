@@ -51,6 +51,7 @@ enum Err : int {
   ERR_INPUT = 256,
   ERR_HITS = 1024,
   ERR_ROUNDS = 2048,
+  ERR_WAIT = 4096,  // a bounded device wait ran out (extension; never expected)
 };
 // metrics[]: the 19 counters of warp_peel/events.py:819-843, then this build's own.
 enum Metric : int {

@@ -1,16 +1,25 @@
 // LSD radix sort, exclusive scan and the Morton hierarchy for gfx950.
 //
 // Radix sort: 8-bit digits, 256-thread blocks (four wave64s) over 4096-key
-// tiles. A pass is three kernels: per-tile digit histograms in LDS; a scan
-// with one block per digit that turns its row of tile counts into offsets
-// within the digit and records the digit's total; and a stable scatter that
-// adds the exclusive scan of the 256 digit totals (computed in LDS by every
-// block) and ranks keys inside the tile with wave64 ballots (eight ballots
-// give each lane the mask of lanes holding the same digit) plus per-wave LDS
-// digit counters. A pass is trivial when one digit holds every key; the
-// scatter then does nothing and the ping-pong parity is tracked on the device,
-// so leading or trailing constant digits cost one histogram pass and no data
-// movement, without any host read.
+// tiles, one kernel per pass with decoupled look-back (Merrill and Garland,
+// "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016; used per
+// radix pass as in Adinets and Merrill, "Onesweep", 2022):
+//   rs_count  one read of the keys counts the digits of every pass;
+//   rs_sweep  a block takes the next tile ticket, ranks its keys by digit with
+//             wave64 ballots (eight ballots give each lane the mask of lanes
+//             holding the same digit) and per-wave LDS counters, publishes its
+//             per-digit counts, adds the counts of the tiles before it by
+//             walking back to the nearest published prefix, publishes its own
+//             prefix and scatters stably;
+//   rs_done   copies back after an odd number of moving passes and clears the
+//             counters, tickets and look-back words for the next sort.
+// Tickets are taken in execution order, so a tile waits only on tiles whose
+// blocks are already running and publish their counts before they wait
+// themselves; every wait is bounded (ERR_WAIT). A pass is trivial when one
+// digit holds every key: it moves nothing and the ping-pong parity follows
+// the non-trivial passes on the device, so leading or trailing constant
+// digits cost no data movement and no host read. Look-back words pack a
+// 2-bit state (0 none, 1 tile count, 2 inclusive prefix) over a 30-bit count.
 #include "primitives.hpp"
 
 namespace peel {
@@ -44,83 +53,65 @@ __device__ __forceinline__ int block_exclusive(int v, int *lds, int &total) {
   return prefix + x - v;
 }
 
-// parity of the non-trivial passes before `pass`
-__device__ __forceinline__ int current_buffer(const int *flags, int pass) {
-  int sel = 0;
-  for (int p = 0; p < pass; ++p) sel ^= flags[p];
-  return sel;
-}
+constexpr unsigned LB_COUNT = 1u << 30, LB_PREFIX = 2u << 30, LB_MASK = (1u << 30) - 1;
+constexpr int LB_SPIN_LIMIT = 1 << 20;
 
-__global__ __launch_bounds__(RS_THREADS) void rs_hist(const uint64_t *k0, const uint64_t *k1,
-                                                       const int *count, int count_static,
-                                                       int capacity, int shift, int pass,
-                                                       int *flags, int *hist, int max_blocks) {
-  __shared__ int bins[256];
+__global__ __launch_bounds__(RS_THREADS) void rs_count(const uint64_t *keys, const int *count, int count_static,
+                                                        int capacity, int first_shift, int passes, int *counts) {
+  __shared__ int bins[RS_MAX_SWEEPS][256];
   int n = live_count(count, count_static, capacity);
   int begin = blockIdx.x * RS_TILE;
-  // non-trivial until the scan finds one digit holding every key
-  if (blockIdx.x == 0 && threadIdx.x == 0) flags[pass] = n > 0 ? 1 : 0;
   if (begin >= n) return;
-  const uint64_t *keys = current_buffer(flags, pass) ? k1 : k0;
-  bins[threadIdx.x] = 0;
+  for (int i = threadIdx.x; i < RS_MAX_SWEEPS * 256; i += RS_THREADS) (&bins[0][0])[i] = 0;
   __syncthreads();
   int end = min(n, begin + RS_TILE);
-  for (int i = begin + threadIdx.x; i < end; i += RS_THREADS)
-    atomicAdd(&bins[(int)((keys[i] >> shift) & 255)], 1);
+  for (int i = begin + threadIdx.x; i < end; i += RS_THREADS) {
+    uint64_t k = keys[i];
+    for (int p = 0; p < passes; ++p) atomicAdd(&bins[p][(int)((k >> (first_shift + 8 * p)) & 255)], 1);
+  }
   __syncthreads();
-  hist[threadIdx.x * max_blocks + blockIdx.x] = bins[threadIdx.x];
+  for (int p = 0; p < passes; ++p)
+    if (bins[p][threadIdx.x]) atomicAdd(&counts[p * 256 + threadIdx.x], bins[p][threadIdx.x]);
 }
 
-// One block per digit: exclusive scan of the digit's tile counts (coalesced
-// chunks of 256) into offsets within the digit, and the digit's total.
-__global__ __launch_bounds__(RS_THREADS) void rs_scan(const int *count, int count_static, int capacity,
-                                                       int pass, int *flags, int *hist, int *totals,
-                                                       int max_blocks) {
-  __shared__ int lds[4];
-  int n = live_count(count, count_static, capacity);
-  int nb = (n + RS_TILE - 1) / RS_TILE;
-  int digit = blockIdx.x;
-  int *row = hist + (long)digit * max_blocks;
-  int running = 0;
-  for (int c = 0; c < nb; c += RS_THREADS) {
-    int b = c + threadIdx.x;
-    int v = b < nb ? row[b] : 0;
-    int total;
-    int excl = block_exclusive(v, lds, total);
-    if (b < nb) row[b] = running + excl;
-    running += total;
-  }
-  if (threadIdx.x == 0) {
-    totals[digit] = running;
-    if (n > 0 && running == n) flags[pass] = 0;
-  }
-}
-
-__global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint64_t *k0, const int *v0,
-                                                          uint64_t *k1, int *v1,
-                                                          const int *count, int count_static,
-                                                          int capacity, int shift, int pass,
-                                                          const int *flags, const int *hist,
-                                                          const int *totals, int max_blocks) {
+__global__ __launch_bounds__(RS_THREADS) void rs_sweep(const uint64_t *k0, const int *v0, uint64_t *k1, int *v1,
+                                                        const int *count, int count_static, int capacity,
+                                                        int shift, int pass, const int *counts, int *tickets,
+                                                        unsigned *tiles, int max_blocks, int *flags,
+                                                        int *status) {
   __shared__ int wave_count[4][256];
   __shared__ int digit_base[256];
   __shared__ int lds[4];
+  __shared__ int trivial, tile_shared;
   int n = live_count(count, count_static, capacity);
-  int tile = blockIdx.x * RS_TILE;
-  if (tile >= n || flags[pass] == 0) return;
-  int sel = current_buffer(flags, pass);
+  // parity of the moving passes before this one, and whether this one moves
+  int sel = 0, moves = 0;
+  for (int q = 0; q <= pass; ++q) {
+    if (threadIdx.x == 0) trivial = 0;
+    __syncthreads();
+    if (counts[q * 256 + threadIdx.x] == n) trivial = 1;
+    __syncthreads();
+    int m = n > 0 && !trivial;
+    __syncthreads();
+    if (q < pass) sel ^= m;
+    else moves = m;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) flags[pass] = moves;
+  if (!moves) return;
+  if (threadIdx.x == 0) tile_shared = atomicAdd(tickets + pass, 1);
+  __syncthreads();
+  const int tile = tile_shared;
+  const int begin = tile * RS_TILE;
+  if (begin >= n) return;
   const uint64_t *kin = sel ? k1 : k0;
   const int *vin = sel ? v1 : v0;
   uint64_t *kout = sel ? (uint64_t *)k0 : k1;
   int *vout = sel ? (int *)v0 : v1;
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < 4 * 256; i += RS_THREADS) (&wave_count[0][0])[i] = 0;
-  int all;
-  int digit_offset = block_exclusive(totals[threadIdx.x], lds, all);
-  digit_base[threadIdx.x] = digit_offset + hist[threadIdx.x * max_blocks + blockIdx.x];
   __syncthreads();
   volatile int *counters = wave_count[wave];
-  int base = tile + wave * 64 * RS_ITEMS;
+  int base = begin + wave * 64 * RS_ITEMS;
   uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t key[RS_ITEMS];
   int val[RS_ITEMS], rank[RS_ITEMS], dig[RS_ITEMS];
@@ -147,14 +138,43 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint64_t *k0, con
     rank[j] = c + below;
   }
   __syncthreads();
-  {
-    int d = threadIdx.x, running = 0;
-    for (int w = 0; w < 4; ++w) {
-      int c = wave_count[w][d];
-      wave_count[w][d] = running;
-      running += c;
-    }
+  // this tile's count of each digit; per-wave offsets inside the tile
+  const int d = threadIdx.x;
+  int tile_count = 0;
+  for (int w = 0; w < 4; ++w) {
+    int c = wave_count[w][d];
+    wave_count[w][d] = tile_count;
+    tile_count += c;
   }
+  unsigned *word = tiles + ((long)pass * max_blocks + tile) * 256 + d;
+  int before = 0;
+  if (tile == 0) {
+    __hip_atomic_store(word, LB_PREFIX | (unsigned)tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    __hip_atomic_store(word, LB_COUNT | (unsigned)tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    for (int t = tile - 1; t >= 0;) {
+      unsigned got = __hip_atomic_load(tiles + ((long)pass * max_blocks + t) * 256 + d, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+      unsigned state = got & ~LB_MASK;
+      if (state == 0) {
+        if (++spins > LB_SPIN_LIMIT) {
+          set_error(status, ERR_WAIT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      before += (int)(got & LB_MASK);
+      if (state == LB_PREFIX) break;
+      --t;
+    }
+    __hip_atomic_store(word, LB_PREFIX | (unsigned)(before + tile_count), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  int all;
+  int digit_start = block_exclusive(counts[pass * 256 + d], lds, all);
+  digit_base[d] = digit_start + before;
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < RS_ITEMS; ++j) {
@@ -167,15 +187,25 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint64_t *k0, con
   }
 }
 
-__global__ void rs_finish(uint64_t *k0, int *v0, const uint64_t *k1, const int *v1,
-                          const int *count, int count_static, int capacity, int passes,
-                          const int *flags) {
+__global__ void rs_done(uint64_t *k0, int *v0, const uint64_t *k1, const int *v1, const int *count,
+                        int count_static, int capacity, int passes, const int *flags, int *counts, int *tickets,
+                        unsigned *tiles, int max_blocks) {
   int n = live_count(count, count_static, capacity);
-  if (!current_buffer(flags, passes)) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    k0[i] = k1[i];
-    v0[i] = v1[i];
-  }
+  int sel = 0;
+  for (int p = 0; p < passes; ++p) sel ^= flags[p];
+  const long stride = (long)gridDim.x * blockDim.x;
+  const long first = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sel)
+    for (long i = first; i < n; i += stride) {
+      k0[i] = k1[i];
+      v0[i] = v1[i];
+    }
+  // leave the scratch zero for the next sort
+  const long used = (long)((n + RS_TILE - 1) / RS_TILE) * 256;
+  for (int p = 0; p < passes; ++p)
+    for (long i = first; i < used; i += stride) tiles[(long)p * max_blocks * 256 + i] = 0u;
+  for (long i = first; i < (long)passes * 256; i += stride) counts[i] = 0;
+  if (first < passes) tickets[first] = 0;
 }
 
 void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int *alt_ids,
@@ -184,21 +214,17 @@ void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int
   int blocks = (capacity + RS_TILE - 1) / RS_TILE;
   if (blocks < 1 || bit_end <= bit_begin) return;
   int first = bit_begin / 8, last = (bit_end + 7) / 8;
-  int passes = 0;
-  for (int digit = first; digit < last; ++digit, ++passes) {
-    int shift = digit * 8;
-    hipLaunchKernelGGL(rs_hist, dim3(blocks), dim3(RS_THREADS), 0, s, keys, alt_keys, count,
-                       count_static, capacity, shift, passes, scratch.flags, scratch.hist,
-                       scratch.max_blocks);
-    hipLaunchKernelGGL(rs_scan, dim3(256), dim3(RS_THREADS), 0, s, count, count_static, capacity, passes,
-                       scratch.flags, scratch.hist, scratch.totals, scratch.max_blocks);
-    hipLaunchKernelGGL(rs_scatter, dim3(blocks), dim3(RS_THREADS), 0, s, keys, ids, alt_keys,
-                       alt_ids, count, count_static, capacity, shift, passes, scratch.flags,
-                       scratch.hist, scratch.totals, scratch.max_blocks);
-  }
+  int passes = last - first;  // at most 8 for 64-bit keys
+  hipLaunchKernelGGL(rs_count, dim3(blocks), dim3(RS_THREADS), 0, s, keys, count, count_static, capacity,
+                     first * 8, passes, scratch.counts);
+  for (int pass = 0; pass < passes; ++pass)
+    hipLaunchKernelGGL(rs_sweep, dim3(blocks), dim3(RS_THREADS), 0, s, keys, ids, alt_keys, alt_ids, count,
+                       count_static, capacity, (first + pass) * 8, pass, scratch.counts, scratch.tickets,
+                       scratch.tiles, scratch.max_blocks, scratch.flags, scratch.status);
   int grid = blocks * 4 < 4096 ? blocks * 4 : 4096;
-  hipLaunchKernelGGL(rs_finish, dim3(grid), dim3(256), 0, s, keys, ids, alt_keys, alt_ids, count,
-                     count_static, capacity, passes, scratch.flags);
+  hipLaunchKernelGGL(rs_done, dim3(grid), dim3(256), 0, s, keys, ids, alt_keys, alt_ids, count, count_static,
+                     capacity, passes, scratch.flags, scratch.counts, scratch.tickets, scratch.tiles,
+                     scratch.max_blocks);
 }
 
 // ---------------------------------------------------------------------------

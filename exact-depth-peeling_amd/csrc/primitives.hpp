@@ -9,12 +9,16 @@ namespace peel {
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per block tile
-constexpr int RS_MAX_PASSES = 16;
+constexpr int RS_MAX_SWEEPS = 8;  // 8-bit digits of a 64-bit key
 
+// Zero between sorts: allocation clears it and every sort's last kernel clears
+// what the sort used.
 struct SortScratch {
-  int *hist = nullptr;   // [256 * max_blocks] digit-major counts / offsets
-  int *flags = nullptr;  // [RS_MAX_PASSES + 1] non-trivial pass flags
-  int *totals = nullptr; // [256] keys per digit of the current pass
+  int *counts = nullptr;      // [RS_MAX_SWEEPS][256] keys per digit of each pass
+  int *tickets = nullptr;     // [RS_MAX_SWEEPS] next tile of each pass
+  unsigned *tiles = nullptr;  // [RS_MAX_SWEEPS][max_blocks][256] look-back words
+  int *flags = nullptr;       // [RS_MAX_SWEEPS] pass moved keys (written by every sort)
+  int *status = nullptr;      // workspace status: a look-back that waits too long sets ERR_WAIT
   int max_blocks = 0;
 };
 

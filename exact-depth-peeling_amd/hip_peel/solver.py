@@ -26,6 +26,7 @@ ERROR_LABELS = {
     64: "layer limit",
     128: "edge noding capacity",
     256: "invalid input",
+    4096: "device wait limit",
 }
 
 
