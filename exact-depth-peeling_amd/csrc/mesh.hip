@@ -402,6 +402,7 @@ constexpr int LANE_PARAMS = 16;
 
 __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
   __shared__ double lds[4][EDGE_MAX];
+  __shared__ int kept_lds[4][EDGE_MAX + 2];  // fan steps the skip rule keeps, then their count
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double *list = lds[wave];
   int n = load_i(w.control + C_CROWDED);
@@ -474,47 +475,63 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
     // The fan steps run across the lanes when no two consecutive parameters
     // (0 and 1 included) lie within 2 EPS: then the sequential skip rule keeps
     // every step, and step s joins the points at parameters s-1 and s, computed
-    // exactly as the sequential walk computes them. Otherwise lane 0 walks.
+    // exactly as the sequential walk computes them. Otherwise lane 0 applies
+    // the skip rule alone (a few LDS reads per step) and the lanes emit the
+    // kept steps, each joining the points of the previous kept parameter and
+    // its own, as the sequential walk does.
     const int steps = total + 1;
     bool tight = false;
     for (int st = lane; st < steps; st += 64) {
       double prev = st == 0 ? 0.0 : list[st - 1], cur = st < total ? list[st] : 1.0;
       tight = tight || cur <= prev + 2.0 * EPS;
     }
-    if (__ballot(tight) != 0) {
-      if (lane == 0) {
-        auto mem = [&](int k) { return list[k]; };
-        int ntri = fan(w, q, center, total, mem, false, 0, 0, 0, sc);
-        int base = atomicAdd(w.status + ST_TRIS, ntri);
-        fan(w, q, center, total, mem, true, base, source, layer, sc);
+    // kept steps: all of them, or those lane 0's pass of the skip rule keeps
+    const bool any_tight = __ballot(tight) != 0;
+    int *kept = kept_lds[wave];
+    if (any_tight && lane == 0) {
+      double previous = 0.0;
+      int m = 0;
+      for (int st = 0; st < steps; ++st) {
+        double following = st < total ? list[st] : 1.0;
+        if (following <= previous + EPS) continue;
+        kept[m++] = st;
+        previous = following;
       }
-    } else {
-      for (int round = 0; round < steps; round += 64) {
-        int st = round + lane;
-        bool emit = false;
-        double2 cursor = q.a, next = q.a;
-        if (st < steps) {
-          double prev = st == 0 ? 0.0 : list[st - 1], cur = st < total ? list[st] : 1.0;
-          if (st > 0) cursor = make_double2(q.a.x + prev * q.d.x, q.a.y + prev * q.d.y);
-          next = make_double2(q.a.x + cur * q.d.x, q.a.y + cur * q.d.y);
-          emit = fabs(cross3(center, cursor, next)) > EPS * EPS;
+      kept[EDGE_MAX + 1] = m;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const int m = any_tight ? kept[EDGE_MAX + 1] : steps;
+    for (int round = 0; round < m; round += 64) {
+      int j = round + lane;
+      bool emit = false;
+      double2 cursor = q.a, next = q.a;
+      if (j < m) {
+        int st = any_tight ? kept[j] : j;
+        double cur = st < total ? list[st] : 1.0;
+        next = make_double2(q.a.x + cur * q.d.x, q.a.y + cur * q.d.y);
+        if (j > 0) {
+          int ps = any_tight ? kept[j - 1] : j - 1;
+          double prev = ps < total ? list[ps] : 1.0;
+          cursor = make_double2(q.a.x + prev * q.d.x, q.a.y + prev * q.d.y);
         }
-        int slot = wave_claim32(w.status + ST_TRIS, emit);
-        if (emit) {
-          if (slot < w.cap.triangles) {
-            double2 tri[3] = {center, cursor, next};
-            double *uv = w.tri_uv + 6L * slot;
-            double *world = w.tri_world + 9L * slot;
-            for (int c = 0; c < 3; ++c) {
-              uv[2 * c] = tri[c].x;
-              uv[2 * c + 1] = tri[c].y;
-              lift(w, tri[c], sc, world + 3 * c);
-            }
-            w.tri_src[slot] = source;
-            w.tri_layer[slot] = layer;
-          } else {
-            set_error(w.status, ERR_TRIS);
+        emit = fabs(cross3(center, cursor, next)) > EPS * EPS;
+      }
+      int slot = wave_claim32(w.status + ST_TRIS, emit);
+      if (emit) {
+        if (slot < w.cap.triangles) {
+          double2 tri[3] = {center, cursor, next};
+          double *uv = w.tri_uv + 6L * slot;
+          double *world = w.tri_world + 9L * slot;
+          for (int c = 0; c < 3; ++c) {
+            uv[2 * c] = tri[c].x;
+            uv[2 * c + 1] = tri[c].y;
+            lift(w, tri[c], sc, world + 3 * c);
           }
+          w.tri_src[slot] = source;
+          w.tri_layer[slot] = layer;
+        } else {
+          set_error(w.status, ERR_TRIS);
         }
       }
     }
