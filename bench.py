@@ -197,6 +197,37 @@ def run_batch(steps, warmup, count=256, seed=0):
     }
 
 
+def run_merged(steps, warmup, count=256, seed=0):
+    """The same 256 icospheres peeled as one workspace over an atlas of their
+    windows (MergedBatch): every kernel runs once over all 6.1 M faces."""
+    import torch
+
+    from hip_peel import MergedBatch, scenes
+
+    meshes = scenes.icosphere_instances(count, seed=seed)
+    wall = time.perf_counter()
+    merged = MergedBatch(meshes, filter_fp32=True)
+    setup_s = time.perf_counter() - wall
+    timing = time_replays(merged.peeler, steps, warmup)
+    info = merged.read_status()
+    faces = sum(len(m[1]) for m in meshes)
+    out = {
+        "instances": count,
+        "faces": faces,
+        "merged_ms": round(timing["median_ms"], 3),
+        "faces_per_s": round(faces / (timing["median_ms"] * 1e-3)),
+        "layers_max": info["layers"],
+        "polygons": info["polygons"],
+        "triangles": info["triangles"],
+        "rebuilt": merged.rebuilt,
+        "pool_gib": round(merged.peeler.device_bytes / 2**30, 3),
+        "setup_s": round(setup_s, 2),
+    }
+    del merged
+    torch.cuda.synchronize()
+    return out
+
+
 def learned_pools(name):
     """Pool memory of the learned sizes against the conservative capacities
     for one workload, with the overflow-rerun outcome and equal outputs."""
@@ -392,6 +423,7 @@ def main():
                            "stages_ms": {k: round(v, 3) for k, v in st.items() if k.startswith("k_")}}
             del p
         extra["icospheres-256"] = run_batch(max(3, args.steps // 2), 1)
+        extra["icospheres-256-merged"] = run_merged(max(3, args.steps // 2), 1)
         result["extra"] = extra
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)

@@ -1,7 +1,7 @@
 """MI355X-native analytic depth peeling: the warp_peel API on gfx950 HIP kernels."""
 
-from .batch import BatchPeeler
+from .batch import BatchPeeler, MergedBatch
 from .events import EventCapacities, EventPeeler
 from .solver import Capacities, DevicePeeler
 
-__all__ = ["BatchPeeler", "Capacities", "DevicePeeler", "EventCapacities", "EventPeeler"]
+__all__ = ["BatchPeeler", "MergedBatch", "Capacities", "DevicePeeler", "EventCapacities", "EventPeeler"]
