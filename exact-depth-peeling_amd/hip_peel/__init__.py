@@ -3,5 +3,6 @@
 from .batch import BatchPeeler, MergedBatch
 from .events import EventCapacities, EventPeeler
 from .solver import Capacities, DevicePeeler
+from .tiles import TiledPeeler
 
-__all__ = ["BatchPeeler", "MergedBatch", "Capacities", "DevicePeeler", "EventCapacities", "EventPeeler"]
+__all__ = ["BatchPeeler", "MergedBatch", "Capacities", "DevicePeeler", "EventCapacities", "EventPeeler", "TiledPeeler"]

@@ -66,6 +66,11 @@ def source_areas(vertices, faces, camera, settings):
     window-clipped source: the area its fragments must tile."""
     cam = vertices[faces] @ camera[:3, :3].T + camera[:3, 3]
     w = settings
+    if not w[4] and len(faces):
+        uv = (cam[..., :2] - w[:2]) / (w[2:4] - w[:2])
+        if uv.min() >= 0.0 and uv.max() <= 1.0:  # nothing clipped: plain triangle areas
+            a, b = uv[:, 1] - uv[:, 0], uv[:, 2] - uv[:, 0]
+            return 0.5 * np.abs(a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0])
     areas = np.zeros(len(faces))
     for f in range(len(faces)):
         ring = [tuple(v) for v in cam[f]]

@@ -133,9 +133,8 @@ def test_random_256_resolution_full_size():
     """The 256-triangle random scene resolved at full size in worker
     processes on the host: 272,806 pieces, each on its parent's plane and
     inside it, and the pieces' areas summing to their parent's (binary64
-    rounding of the exact pieces only). Peeling it needs more than 16 times
-    the default pools (over 25 million fragments), so only the resolution
-    runs here."""
+    rounding of the exact pieces only). Its whole window holds about 10^9
+    event fragments; test_gpu_tiles peels it inside a window at its densest."""
     from hip_peel import scenes
 
     print("[resolved] random 256 resolution", flush=True)
