@@ -55,19 +55,23 @@ __device__ __forceinline__ int block_exclusive(int v, int *lds, int &total) {
 
 constexpr unsigned LB_COUNT = 1u << 30, LB_PREFIX = 2u << 30, LB_MASK = (1u << 30) - 1;
 constexpr int LB_SPIN_LIMIT = 1 << 20;
+constexpr int LB_BATCH = 8;   // predecessor words loaded together in the look-back
+constexpr int RC_TILE = 1024;  // keys per block of the digit count (independent of the sweep's tiles)
 
 __global__ __launch_bounds__(RS_THREADS) void rs_count(const uint64_t *keys, const int *count, int count_static,
                                                         int capacity, int first_shift, int passes, int *counts) {
   __shared__ int bins[RS_MAX_SWEEPS][256];
   int n = live_count(count, count_static, capacity);
-  int begin = blockIdx.x * RS_TILE;
-  if (begin >= n) return;
+  // chunks of RC_TILE keys, grid-strided over at most one block per CU
+  if ((long)blockIdx.x * RC_TILE >= n) return;
   for (int i = threadIdx.x; i < RS_MAX_SWEEPS * 256; i += RS_THREADS) (&bins[0][0])[i] = 0;
   __syncthreads();
-  int end = min(n, begin + RS_TILE);
-  for (int i = begin + threadIdx.x; i < end; i += RS_THREADS) {
-    uint64_t k = keys[i];
-    for (int p = 0; p < passes; ++p) atomicAdd(&bins[p][(int)((k >> (first_shift + 8 * p)) & 255)], 1);
+  for (long begin = (long)blockIdx.x * RC_TILE; begin < n; begin += (long)gridDim.x * RC_TILE) {
+    long end = min((long)n, begin + RC_TILE);
+    for (long i = begin + threadIdx.x; i < end; i += RS_THREADS) {
+      uint64_t k = keys[i];
+      for (int p = 0; p < passes; ++p) atomicAdd(&bins[p][(int)((k >> (first_shift + 8 * p)) & 255)], 1);
+    }
   }
   __syncthreads();
   for (int p = 0; p < passes; ++p)
@@ -84,12 +88,18 @@ __global__ __launch_bounds__(RS_THREADS) void rs_sweep(const uint64_t *k0, const
   __shared__ int lds[4];
   __shared__ int trivial, tile_shared;
   int n = live_count(count, count_static, capacity);
+  // this digit's count in every pass so far, loaded together
+  int digit_counts[RS_MAX_SWEEPS];
+#pragma unroll
+  for (int q = 0; q < RS_MAX_SWEEPS; ++q) digit_counts[q] = q <= pass ? counts[q * 256 + threadIdx.x] : 0;
   // parity of the moving passes before this one, and whether this one moves
   int sel = 0, moves = 0;
-  for (int q = 0; q <= pass; ++q) {
+#pragma unroll
+  for (int q = 0; q < RS_MAX_SWEEPS; ++q) {
+    if (q > pass) break;
     if (threadIdx.x == 0) trivial = 0;
     __syncthreads();
-    if (counts[q * 256 + threadIdx.x] == n) trivial = 1;
+    if (digit_counts[q] == n) trivial = 1;
     __syncthreads();
     int m = n > 0 && !trivial;
     __syncthreads();
@@ -98,6 +108,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_sweep(const uint64_t *k0, const
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) flags[pass] = moves;
   if (!moves) return;
+  // only as many blocks as there are tiles take a ticket (blocks 0 .. tiles-1
+  // take tickets 0 .. tiles-1, in execution order); the others leave at once
+  if ((long)blockIdx.x * RS_TILE >= n) return;
   if (threadIdx.x == 0) tile_shared = atomicAdd(tickets + pass, 1);
   __syncthreads();
   const int tile = tile_shared;
@@ -115,12 +128,19 @@ __global__ __launch_bounds__(RS_THREADS) void rs_sweep(const uint64_t *k0, const
   uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t key[RS_ITEMS];
   int val[RS_ITEMS], rank[RS_ITEMS], dig[RS_ITEMS];
+  // every load of the tile first, so that they are all in flight together
+  // (the ranking below orders itself through the LDS counters)
 #pragma unroll
   for (int j = 0; j < RS_ITEMS; ++j) {
     int idx = base + j * 64 + lane;
     bool valid = idx < n;
     key[j] = valid ? kin[idx] : 0ull;
     val[j] = valid ? vin[idx] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; ++j) {
+    int idx = base + j * 64 + lane;
+    bool valid = idx < n;
     int d = valid ? (int)((key[j] >> shift) & 255) : 0;
     dig[j] = d;
     uint64_t peers = __ballot(valid);
@@ -152,28 +172,55 @@ __global__ __launch_bounds__(RS_THREADS) void rs_sweep(const uint64_t *k0, const
     __hip_atomic_store(word, LB_PREFIX | (unsigned)tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   } else {
     __hip_atomic_store(word, LB_COUNT | (unsigned)tile_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // The words of up to LB_BATCH predecessors are loaded together and
+    // consumed nearest first: a prefix ends the walk, a count is added and
+    // the walk goes on, an unpublished word is loaded again (with the rest
+    // of its batch). The sum is the one a word-by-word walk finds, with one
+    // load latency per batch instead of per predecessor.
     int spins = 0;
+    const unsigned *column = tiles + (long)pass * max_blocks * 256 + d;
     for (int t = tile - 1; t >= 0;) {
-      unsigned got = __hip_atomic_load(tiles + ((long)pass * max_blocks + t) * 256 + d, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-      unsigned state = got & ~LB_MASK;
-      if (state == 0) {
+      unsigned got[LB_BATCH];
+#pragma unroll
+      for (int j = 0; j < LB_BATCH; ++j)
+        got[j] = t - j >= 0 ? __hip_atomic_load(column + (long)(t - j) * 256, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                            : LB_PREFIX;
+      bool done = false, waiting = false;
+#pragma unroll
+      for (int j = 0; j < LB_BATCH; ++j) {
+        if (done || waiting) continue;
+        unsigned state = got[j] & ~LB_MASK;
+        if (state == 0) {
+          waiting = true;
+          continue;
+        }
+        if (t < 0) {  // past tile 0 (only reached after tile 0's prefix)
+          done = true;
+          continue;
+        }
+        before += (int)(got[j] & LB_MASK);
+        if (state == LB_PREFIX) done = true;
+        --t;
+      }
+      if (done) break;
+      if (waiting) {
         if (++spins > LB_SPIN_LIMIT) {
           set_error(status, ERR_WAIT);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      before += (int)(got & LB_MASK);
-      if (state == LB_PREFIX) break;
-      --t;
     }
     __hip_atomic_store(word, LB_PREFIX | (unsigned)(before + tile_count), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   }
   int all;
-  int digit_start = block_exclusive(counts[pass * 256 + d], lds, all);
+  int this_pass = 0;
+#pragma unroll
+  for (int q = 0; q < RS_MAX_SWEEPS; ++q)
+    if (q == pass) this_pass = digit_counts[q];
+  int digit_start = block_exclusive(this_pass, lds, all);
   digit_base[d] = digit_start + before;
   __syncthreads();
 #pragma unroll
@@ -215,7 +262,9 @@ void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int
   if (blocks < 1 || bit_end <= bit_begin) return;
   int first = bit_begin / 8, last = (bit_end + 7) / 8;
   int passes = last - first;  // at most 8 for 64-bit keys
-  hipLaunchKernelGGL(rs_count, dim3(blocks), dim3(RS_THREADS), 0, s, keys, count, count_static, capacity,
+  int count_blocks = (capacity + RC_TILE - 1) / RC_TILE;
+  if (count_blocks > 256) count_blocks = 256;
+  hipLaunchKernelGGL(rs_count, dim3(count_blocks), dim3(RS_THREADS), 0, s, keys, count, count_static, capacity,
                      first * 8, passes, scratch.counts);
   for (int pass = 0; pass < passes; ++pass)
     hipLaunchKernelGGL(rs_sweep, dim3(blocks), dim3(RS_THREADS), 0, s, keys, ids, alt_keys, alt_ids, count,
