@@ -91,9 +91,8 @@ enum Control : int {
   C_HEAVY = 96,     // heavy ray jobs
   C_NHITS = 128,    // stored hits (clamped to capacity)
   C_NBV = 129,      // boundary vertices (scan total)
-  C_SORTN = 130,    // hits handed to the global sort (0 unless a huge group exists)
   C_MEDIUM = 131,   // groups sorted by a workgroup
-  C_HUGE = 132,     // any group too large for a workgroup sort
+  C_HUGE = 132,     // groups too large for one LDS sort (listed from the end of `medium`)
   C_NSEG = 133,     // unique contour segments
   C_INTERNAL = 135, // any internal (non-contour) mesh edge
   C_CONNECT = 136,  // connectivity pass enabled for this run
@@ -104,6 +103,7 @@ enum Control : int {
   C_LEADERS = 141,   // ray-group leaders (they sort first in the ray order)
   C_GRID = 142,      // cells per axis of the boundary-vertex grid
   C_LIVE = 143,      // patches left after convex coalescing
+  C_HUGE_SYNC = 160, // arrivals at the grid barriers of the huge-group sort
   C_COUNT = 256,
 };
 

@@ -48,6 +48,7 @@ __global__ void k_rank_reset(Ws w) {
     w.control[C_HEAVY] = 0;
     w.control[C_MEDIUM] = 0;
     w.control[C_HUGE] = 0;
+    w.control[C_HUGE_SYNC] = 0;
     w.control[C_LEADERS] = 0;
   }
 }
@@ -575,32 +576,12 @@ __global__ void k_check_hits(Ws w) {
   w.control[C_NHITS] = (int)(h < w.cap.hits ? h : w.cap.hits);
 }
 
-// Stable LSD stages: source, then depth, then group => (group, depth, source).
-__device__ void hit_key_one(const Ws &w, int stage, int i);
-
-__global__ void k_hit_keys(Ws w, int stage) {
-  ITEM_LOOP(i, load_i(w.control + C_SORTN)) hit_key_one(w, stage, i);
-}
-
-__device__ void hit_key_one(const Ws &w, int stage, int i) {
-  int h = stage == 0 ? i : w.hids[i];
-  if (stage == 0) w.hids[i] = i;
-  uint64_t key;
-  if (stage == 0)
-    key = (uint32_t)w.hit_src[h];
-  else if (stage == 1)
-    key = dkey(w.hit_depth[h]);
-  else
-    key = (uint32_t)w.hit_group[h];
-  w.hkeys[i] = key;
-}
-
 // ---------------------------------------------------------------------------
 // Hit order (group, depth, source). Each group's hits form one contiguous
 // segment at its scanned offset; segments are sorted independently: up to 64
 // hits by one wave (bitonic network across lanes, in registers), up to 2048 by
-// one workgroup (bitonic in LDS). Only a larger group (e.g. 65,536 identical
-// footprints) enables the global three-stage radix sort over all hits.
+// one workgroup (bitonic in LDS), larger ones by one workgroup each with
+// merge passes (k_sort_huge).
 constexpr int SMALL_SEGMENT = 64, MEDIUM_SEGMENT = 2048;
 
 __device__ __forceinline__ bool hit_less(uint64_t ka, int sa, uint64_t kb, int sb) {
@@ -635,7 +616,10 @@ __device__ void sort_small_one(const Ws &w, int g) {
         int k = atomicAdd(w.control + C_MEDIUM, 1);
         w.medium[k] = g;
       } else {
-        atomicOr(w.control + C_HUGE, 1);
+        // huge groups fill the same list from its end (a group is one or
+        // the other, so the two ends never meet)
+        int k = atomicAdd(w.control + C_HUGE, 1);
+        w.medium[w.cap.regions - 1 - k] = g;
       }
     }
     return;
@@ -717,8 +701,181 @@ __global__ __launch_bounds__(256) void k_sort_medium(Ws w) {
   }
 }
 
-__global__ void k_sort_plan(Ws w) {
-  w.control[C_SORTN] = w.control[C_HUGE] ? w.control[C_NHITS] : 0;
+// Groups above MEDIUM_SEGMENT hits (e.g. 65,536 identical footprints) are
+// sorted by the HUGE_BLOCKS workgroups of one launch together: chunks of
+// MEDIUM_SEGMENT bitonic-sorted in LDS (spread over the blocks), then merge
+// passes of doubling run width, each grid thread producing a contiguous span
+// of outputs from its merge-path split, with a grid barrier after every
+// phase. A group's passes alternate between (hkeys, hids) and (hkeys_alt,
+// hids_alt); an odd count leaves its order in hids_alt, copied back at the
+// end. Sources are unique within a group, so the order (depth key, source) is
+// total and equals the small and medium sorts'. With no huge group every
+// block returns at once.
+constexpr int HUGE_THREADS = 1024, HUGE_BLOCKS = 64;
+
+__device__ __forceinline__ bool hit_before(const Ws &w, uint64_t ka, int ha, uint64_t kb, int hb) {
+  return ka < kb || (ka == kb && w.hit_src[ha] < w.hit_src[hb]);
+}
+
+// Grid barrier of k_sort_huge: its HUGE_BLOCKS blocks are all resident (the
+// launch is small and follows the previous kernel of its stream), every block
+// passes the same barriers, and a bounded wait sets ERR_WAIT instead of
+// hanging. Agent-scope fences publish the blocks' writes across the chip.
+__device__ void huge_barrier(const Ws &w, int &generation) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    generation += 1;
+    __threadfence();
+    atomicAdd(w.control + C_HUGE_SYNC, 1);
+    const int target = generation * (int)gridDim.x;
+    int spins = 0;
+    while (__hip_atomic_load(w.control + C_HUGE_SYNC, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++spins > (1 << 20)) {
+        set_error(w.status, ERR_WAIT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(HUGE_THREADS) void k_sort_huge(Ws w) {
+  __shared__ unsigned long long kk[MEDIUM_SEGMENT];
+  __shared__ int ss[MEDIUM_SEGMENT], hh[MEDIUM_SEGMENT];
+  if (w.status[ST_ERR] != 0) return;
+  const int count = load_i(w.control + C_HUGE);
+  if (count == 0) return;
+  int generation = 0;
+  // 1) every chunk of every huge group, one LDS sort each, into (hkeys, hids)
+  long tasks = 0, largest = 0;
+  for (int item = 0; item < count; ++item) {
+    int c = w.group_counts[w.medium[w.cap.regions - 1 - item]];
+    tasks += (c + MEDIUM_SEGMENT - 1) / MEDIUM_SEGMENT;
+    largest = max(largest, (long)c);
+  }
+  for (long task = blockIdx.x; task < tasks; task += gridDim.x) {
+    int item = 0;
+    long first = 0;
+    int g = 0, c = 0;
+    for (;; ++item) {
+      g = w.medium[w.cap.regions - 1 - item];
+      c = w.group_counts[g];
+      long chunks = (c + MEDIUM_SEGMENT - 1) / MEDIUM_SEGMENT;
+      if (task < first + chunks) break;
+      first += chunks;
+    }
+    const int off = w.group_offsets[g];
+    const int base = (int)(task - first) * MEDIUM_SEGMENT;
+    const int m = min(MEDIUM_SEGMENT, c - base);
+    int n2 = 1;
+    while (n2 < m) n2 <<= 1;
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+      if (i < m) {
+        int h = w.hids_alt[off + base + i];
+        kk[i] = dkey(w.hit_depth[h]);
+        ss[i] = w.hit_src[h];
+        hh[i] = h;
+      } else {
+        kk[i] = ~0ull;
+        ss[i] = 0x7fffffff;
+        hh[i] = -1;
+      }
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+          int p = i ^ j;
+          if (p > i) {
+            bool up = (i & k) == 0;
+            bool greater = hit_less(kk[p], ss[p], kk[i], ss[i]);
+            if (greater == up) {
+              unsigned long long tk = kk[i];
+              kk[i] = kk[p];
+              kk[p] = tk;
+              int t = ss[i];
+              ss[i] = ss[p];
+              ss[p] = t;
+              t = hh[i];
+              hh[i] = hh[p];
+              hh[p] = t;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      w.hkeys[off + base + i] = kk[i];
+      w.hids[off + base + i] = hh[i];
+    }
+    __syncthreads();
+  }
+  huge_barrier(w, generation);
+  // 2) merge passes over the whole grid
+  const long threads = (long)gridDim.x * blockDim.x;
+  const long me = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int pass = 0;
+  for (long width = MEDIUM_SEGMENT; width < largest; width *= 2, ++pass) {
+    for (int item = 0; item < count; ++item) {
+      const int g = w.medium[w.cap.regions - 1 - item];
+      const long c = w.group_counts[g];
+      if (c <= width) continue;
+      const int off = w.group_offsets[g];
+      const uint64_t *kin = (pass & 1 ? w.hkeys_alt : w.hkeys) + off;
+      const int *iin = (pass & 1 ? w.hids_alt : w.hids) + off;
+      uint64_t *kout = (pass & 1 ? w.hkeys : w.hkeys_alt) + off;
+      int *iout = (pass & 1 ? w.hids : w.hids_alt) + off;
+      const long per = (c + threads - 1) / threads;
+      long k = me * per;
+      const long k_end = min(c, k + per);
+      while (k < k_end) {
+        const long a = k / (2 * width) * (2 * width);
+        const long mid_run = min(c, a + width), e = min(c, a + 2 * width);
+        const long nl = mid_run - a, nr = e - mid_run;
+        const uint64_t *lk = kin + a, *rk = kin + mid_run;
+        const int *li = iin + a, *ri = iin + mid_run;
+        // merge-path split of the first d outputs of this pair of runs
+        const long d = k - a;
+        long lo = max(0L, d - nr), hi = min(d, nl);
+        while (lo < hi) {
+          long m = (lo + hi) / 2;
+          if (hit_before(w, lk[m], li[m], rk[d - m - 1], ri[d - m - 1]))
+            lo = m + 1;
+          else
+            hi = m;
+        }
+        long i = lo, j = d - lo;
+        const long stop = min(k_end, e);
+        for (; k < stop; ++k) {
+          bool left = j >= nr || (i < nl && hit_before(w, lk[i], li[i], rk[j], ri[j]));
+          if (left) {
+            kout[k] = lk[i];
+            iout[k] = li[i];
+            ++i;
+          } else {
+            kout[k] = rk[j];
+            iout[k] = ri[j];
+            ++j;
+          }
+        }
+      }
+    }
+    huge_barrier(w, generation);
+  }
+  // 3) a group with an odd number of passes holds its order in hids_alt
+  for (int item = 0; item < count; ++item) {
+    const int g = w.medium[w.cap.regions - 1 - item];
+    const long c = w.group_counts[g];
+    int passes = 0;
+    for (long width = MEDIUM_SEGMENT; width < c; width *= 2) ++passes;
+    if (passes & 1) {
+      const int off = w.group_offsets[g];
+      for (long i = me; i < c; i += threads) w.hids[off + i] = w.hids_alt[off + i];
+    }
+  }
 }
 
 __device__ int assign_one(const Ws &w, int i);
@@ -799,16 +956,7 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   hipLaunchKernelGGL(k_hit_scatter, stride_grid(w.cap.hits), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_sort_small, stride_grid(w.cap.regions, 4), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_sort_medium, dim3(1024), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_sort_plan, dim3(1), dim3(1), 0, s, w);
-  const int *nh = w.control + C_SORTN;
-  int src_bits = 1, group_bits = 1;
-  while ((1L << src_bits) < (long)w.nf) ++src_bits;
-  while ((1L << group_bits) < (long)w.cap.regions) ++group_bits;
-  for (int stage = 0; stage < 3; ++stage) {
-    hipLaunchKernelGGL(k_hit_keys, stride_grid(w.cap.hits), dim3(256), 0, s, w, stage);
-    int bits = stage == 0 ? src_bits : (stage == 1 ? 64 : group_bits);
-    radix_sort(s, w.hkeys, w.hids, w.hkeys_alt, w.hids_alt, nh, 0, w.cap.hits, 0, bits, w.sort);
-  }
+  hipLaunchKernelGGL(k_sort_huge, dim3(HUGE_BLOCKS), dim3(HUGE_THREADS), 0, s, w);
   mark(t, s, "hit_sort");
   hipLaunchKernelGGL(k_assign, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   mark(t, s, "k_assign");

@@ -443,6 +443,28 @@ def test_stack_hash_is_bounded():
     np.testing.assert_array_equal(p.output_layers[:4096].cpu().numpy(), p.output_sources[:4096].cpu().numpy())
 
 
+@pytest.mark.parametrize("n", [2049, 5000])
+def test_huge_group_order(n):
+    """One ray group larger than a workgroup's LDS sort (2,048 hits): the
+    stack's depths in shuffled order, so the merge passes must order the hits
+    by depth for every layer to equal the source's depth rank."""
+    import torch
+
+    from hip_peel import EventPeeler
+
+    arrays = stack(n)
+    perm = np.random.default_rng(n).permutation(n)
+    xyz = arrays[0].cpu().numpy().reshape(n, 3, 3)
+    xyz[:, :, 2] = (perm + 1)[:, None]
+    arrays[0] = torch.tensor(xyz.reshape(-1, 3), device="cuda")
+    p = EventPeeler(*arrays)
+    p.run()
+    info = p.read_status()
+    assert info["polygons"] == n and info["ray_groups"] == 1 and info["layers"] == n
+    src = p.output_sources[:n].cpu().numpy()
+    np.testing.assert_array_equal(p.output_layers[:n].cpu().numpy(), perm[src])
+
+
 def test_query_sort_bucket_replay():
     """test_components.py:96-114: one captured graph, changing polygon counts."""
     import torch
