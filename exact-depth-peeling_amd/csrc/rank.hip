@@ -580,20 +580,42 @@ __global__ void k_check_hits(Ws w) {
 // Hit order (group, depth, source). Each group's hits form one contiguous
 // segment at its scanned offset; segments are sorted independently: up to 64
 // hits by one wave (bitonic network across lanes, in registers), up to 2048 by
-// one workgroup (bitonic in LDS), larger ones by one workgroup each with
-// merge passes (k_sort_huge).
+// one workgroup (bitonic in LDS), larger ones by the workgroups of
+// k_sort_huge together (LDS chunks, then merge passes).
 constexpr int SMALL_SEGMENT = 64, MEDIUM_SEGMENT = 2048;
 
 __device__ __forceinline__ bool hit_less(uint64_t ka, int sa, uint64_t kb, int sb) {
   return ka < kb || (ka == kb && sa < sb);
 }
 
+// Hits into their group's segment. The order inside a segment does not
+// matter (the segment sorts are total). A ray's hits sit next to each other,
+// so the lanes of a wave that share the first lane's group take their slots
+// with one atomic (one group of 65,536 hits would otherwise serialise 65,536
+// atomics on one counter); the other lanes, usually of small groups, take
+// theirs one by one.
 __global__ void k_hit_scatter(Ws w) {
   if (w.status[ST_ERR] != 0) return;
-  ITEM_LOOP(h, load_i(w.control + C_NHITS)) {
-    int g = w.hit_group[h];
-    int pos = w.group_offsets[g] + atomicAdd(&w.group_fill[g], 1);
-    w.hids_alt[pos] = h;
+  const int n = load_i(w.control + C_NHITS);
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1;
+  WAVE_LOOP(base, n) {
+    int h = base + threadIdx.x;
+    bool valid = h < n;
+    int g = valid ? w.hit_group[h] : -1;
+    int leader = __ffsll((unsigned long long)__ballot(valid)) - 1;  // lane 0 unless the wave is empty
+    int lg = __shfl(g, leader < 0 ? 0 : leader, 64);
+    bool shared = valid && g == lg;
+    uint64_t peers = __ballot(shared);
+    int first = 0;
+    if (lane == leader && peers) first = atomicAdd(&w.group_fill[lg], __popcll(peers));
+    first = __shfl(first, leader < 0 ? 0 : leader, 64);
+    int pos = 0;
+    if (shared)
+      pos = w.group_offsets[lg] + first + __popcll(peers & below);
+    else if (valid)
+      pos = w.group_offsets[g] + atomicAdd(&w.group_fill[g], 1);
+    if (valid) w.hids_alt[pos] = h;
   }
 }
 
