@@ -159,8 +159,12 @@ __device__ __forceinline__ double unkey(uint64_t k) {
 }
 
 // Atomic min on binary64 through the order-preserving key (values are finite).
+// The word only decreases, so an update that cannot lower the value it reads
+// is skipped: many copies of one key (65,536 identical footprints share three
+// segments) then cost a load each instead of an atomic on one address.
 __device__ __forceinline__ void atomic_min_key(unsigned long long *p, double x) {
-  atomicMin(p, (unsigned long long)dkey(x));
+  unsigned long long k = (unsigned long long)dkey(x);
+  if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > k) atomicMin(p, k);
 }
 
 // Static forward error bound of a binary32 orientation test

@@ -431,13 +431,46 @@ __global__ void k_segments(Ws w) {
 // Exact duplicate grouping: the minimum slot of equal segments owns the hash
 // slot. The endpoints compared were written by k_segments (a previous
 // kernel), so the probe needs no fences.
+// Copies of one segment in a wave (the same starting hash slot and the same
+// endpoints, e.g. the three edges shared by 65,536 identical footprints)
+// act through their lowest lane, which also holds the smallest index: up to
+// SEGMENT_ROUNDS such groups per wave, each found by one ballot. `leader` is
+// the lane that acts for this lane (itself when ungrouped).
+constexpr int SEGMENT_ROUNDS = 4;
+
+__device__ __forceinline__ uint64_t segment_group(bool valid, int slot, double2 a, double2 b, int &leader,
+                                                  int round_leader[SEGMENT_ROUNDS]) {
+  const int lane = threadIdx.x & 63;
+  leader = lane;
+  uint64_t todo = __ballot(valid);
+  uint64_t grouped = 0;
+  for (int r = 0; r < SEGMENT_ROUNDS; ++r) {
+    round_leader[r] = -1;
+    if (!todo) continue;
+    int l = __ffsll((unsigned long long)todo) - 1;
+    int ls = __shfl(slot, l, 64);
+    double2 la = make_double2(__shfl(a.x, l, 64), __shfl(a.y, l, 64));
+    double2 lb = make_double2(__shfl(b.x, l, 64), __shfl(b.y, l, 64));
+    bool peer = valid && ((grouped >> lane) & 1) == 0 && slot == ls && same(a, la) && same(b, lb);
+    uint64_t peers = __ballot(peer);
+    if (peer) leader = l;
+    round_leader[r] = l;
+    grouped |= peers;
+    todo &= ~peers;
+  }
+  return grouped;
+}
+
 __global__ void k_segments_insert(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.nseg) return;
-  double2 a = w.seg_a[i], b = w.seg_b[i];
-  if (a.x >= 1e29) return;
+  bool valid = i < w.nseg;
+  double2 a = valid ? w.seg_a[i] : make_double2(1e30, 0), b = valid ? w.seg_b[i] : make_double2(0, 0);
+  valid = valid && a.x < 1e29;
   int mask = w.shash_size - 1;
-  int slot = (int)(segment_hash(a, b) & (uint64_t)mask);
+  int slot = valid ? (int)(segment_hash(a, b) & (uint64_t)mask) : -1;
+  int leader, rounds[SEGMENT_ROUNDS];
+  segment_group(valid, slot, a, b, leader, rounds);
+  if (!valid || leader != (int)(threadIdx.x & 63)) return;
   for (int probe = 0; probe < w.shash_size; ++probe) {
     int cur = load_i(&w.shash[slot]);
     if (cur < 0) {
@@ -455,27 +488,50 @@ __global__ void k_segments_insert(Ws w) {
 
 __global__ void k_segments_reduce(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= w.nseg) return;
-  double2 a = w.seg_a[i], b = w.seg_b[i];
-  if (a.x >= 1e29) return;
+  bool valid = i < w.nseg;
+  double2 a = valid ? w.seg_a[i] : make_double2(1e30, 0), b = valid ? w.seg_b[i] : make_double2(0, 0);
+  valid = valid && a.x < 1e29;
   int mask = w.shash_size - 1;
-  int slot = (int)(segment_hash(a, b) & (uint64_t)mask);
+  int slot = valid ? (int)(segment_hash(a, b) & (uint64_t)mask) : -1;
+  int leader, rounds[SEGMENT_ROUNDS];
+  segment_group(valid, slot, a, b, leader, rounds);
   int rep = i;
-  for (int probe = 0; probe < w.shash_size; ++probe) {
-    int cur = w.shash[slot];
-    if (cur < 0) break;
-    if (same(w.seg_a[cur], a) && same(w.seg_b[cur], b)) {
-      rep = cur;
-      break;
+  unsigned long long ka = ~0ull, kb = ~0ull;
+  if (valid) {
+    int probe_slot = slot;
+    for (int probe = 0; probe < w.shash_size; ++probe) {
+      int cur = w.shash[probe_slot];
+      if (cur < 0) break;
+      if (same(w.seg_a[cur], a) && same(w.seg_b[cur], b)) {
+        rep = cur;
+        break;
+      }
+      probe_slot = (probe_slot + 1) & mask;
     }
-    slot = (slot + 1) & mask;
+    w.seg_rep[i] = rep;
+    double4 sc = w.scores[i / 8];
+    ka = (unsigned long long)dkey(value(sc, a));
+    kb = (unsigned long long)dkey(value(sc, b));
   }
-  w.seg_rep[i] = rep;
-  int source = i / 8;
-  double4 sc = w.scores[source];
-  atomic_min_key(&w.seg_near[rep], value(sc, a));
-  atomic_min_key(&w.seg_near[w.nseg + rep], value(sc, b));
-  if (rep == i) atomicAdd((unsigned long long *)&w.metrics[M_UNIQUE], 1ull);
+  // each group's nearest endpoint depths, gathered on its leader
+  const int lane = threadIdx.x & 63;
+  for (int r = 0; r < SEGMENT_ROUNDS; ++r) {
+    if (rounds[r] < 0) continue;  // wave-uniform
+    bool peer = valid && leader == rounds[r];
+    unsigned long long ma = peer ? ka : ~0ull, mb = peer ? kb : ~0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+      ma = min(ma, (unsigned long long)__shfl_xor(ma, off, 64));
+      mb = min(mb, (unsigned long long)__shfl_xor(mb, off, 64));
+    }
+    if (lane == rounds[r]) ka = ma, kb = mb;
+  }
+  if (valid && leader == lane) {
+    if (__hip_atomic_load(&w.seg_near[rep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > ka)
+      atomicMin(&w.seg_near[rep], ka);
+    if (__hip_atomic_load(&w.seg_near[w.nseg + rep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > kb)
+      atomicMin(&w.seg_near[w.nseg + rep], kb);
+  }
+  if (valid && rep == i) atomicAdd((unsigned long long *)&w.metrics[M_UNIQUE], 1ull);
 }
 
 __global__ void k_segment_flags(Ws w) {
