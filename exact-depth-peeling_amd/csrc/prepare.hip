@@ -206,6 +206,82 @@ __device__ void ring_keep(Ring8 &p, double sx, double sy, double sc) {
   p = in;
 }
 
+// The common case of k_project: a triangle in front of the near plane whose
+// projection lies inside the window. Every clip of the general path then
+// keeps the ring as it is, and its cleaning reduces to the checks below, so
+// this computes the same values with the three vertices in registers (the
+// general path's run-time indexed rings live in scratch memory). Returns
+// false, having written nothing, when a vertex leaves the window.
+__device__ __forceinline__ bool project_unclipped(const Ws &w, int i, const double (&c)[3][3], bool persp) {
+  const double *s = w.settings;
+  double2 v[3];
+  double depth[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double invz = 1.0;
+    if (persp) invz = 1.0 / c[j][2];
+    v[j] = make_double2((c[j][0] * invz - s[0]) / (s[2] - s[0]), (c[j][1] * invz - s[1]) / (s[3] - s[1]));
+    depth[j] = persp ? -invz : c[j][2];
+    if (!(v[j].x >= 0.0 && v[j].x <= 1.0 && v[j].y >= 0.0 && v[j].y <= 1.0)) return false;
+  }
+  const int nf = w.nf;
+  double4 score = make_double4(0, 0, 0, 0);
+  double det = cross3(v[0], v[1], v[2]);
+  int face = 0, n = 0;
+  double2 u[3] = {v[0], v[1], v[2]};
+  if (det != 0.0) {
+    double2 pa = make_double2(v[1].x - v[0].x, v[1].y - v[0].y);
+    double2 pb = make_double2(v[2].x - v[0].x, v[2].y - v[0].y);
+    double s0 = ((depth[1] - depth[0]) * pb.y - (depth[2] - depth[0]) * pa.y) / det;
+    double s1 = (pa.x * (depth[2] - depth[0]) - pb.x * (depth[1] - depth[0])) / det;
+    score = make_double4(s0, s1, depth[0] - s0 * v[0].x - s1 * v[0].y, 0.0);
+    face = 1;
+    if (det < 0.0) {
+      face = -1;
+      u[0] = v[2];
+      u[2] = v[0];
+    }
+    // ring_clean of a triangle: consecutive and closing vertices apart, then
+    // the area test, summed in ring order as ring_clean sums it
+    const double gap = EPS * 0.01;
+    bool apart = dlength(make_double2(u[1].x - u[0].x, u[1].y - u[0].y)) > gap &&
+                 dlength(make_double2(u[2].x - u[1].x, u[2].y - u[1].y)) > gap &&
+                 !(dlength(make_double2(u[0].x - u[2].x, u[0].y - u[2].y)) <= gap);
+    if (apart) {
+      double area2 = 0.0, perimeter = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double2 a = u[j], b = u[(j + 1) % 3];
+        area2 += cross3(u[0], a, b);
+        perimeter += fabs(b.x - a.x) + fabs(b.y - a.y);
+      }
+      if (!(area2 <= EPS * perimeter)) n = 3;
+    }
+  }
+  if (!isfinite(score.x) || !isfinite(score.y) || !isfinite(score.z) || fabs(score.x) > 1e28 ||
+      fabs(score.y) > 1e28 || fabs(score.z) > 1e28)
+    set_error(w.status, ERR_INPUT);
+  w.facing[i] = face;
+  w.clipped[i] = 0;
+  w.scores[i] = score;
+  w.src_n[i] = n;
+  double lx = 1e30, ly = 1e30, lz = 1e30, hx = -1e30, hy = -1e30, hz = -1e30;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    if (j < n) {
+      double2 x = u[j];
+      w.src_xy[j * nf + i] = x;
+      double z = value(score, x);
+      lx = fmin(lx, x.x), ly = fmin(ly, x.y), lz = fmin(lz, z);
+      hx = fmax(hx, x.x), hy = fmax(hy, x.y), hz = fmax(hz, z);
+    }
+  }
+  Box b;
+  pad_box(lx, ly, lz, hx, hy, hz, b.lo, b.hi);
+  w.src_boxes[i] = b;
+  return true;
+}
+
 __global__ void k_project(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= w.nf) return;
@@ -224,18 +300,28 @@ __global__ void k_project(Ws w) {
   const double *m = w.camera;
   const double *s = w.settings;
   bool persp = s[4] != 0.0;
-  double r[SRCV][3];
-  int count = 3;
+  double c[3][3];
   bool bad = false;
+#pragma unroll
   for (int j = 0; j < 3; ++j) {
     int vi = w.faces[3 * i + j];
     double x = w.vertices[3 * vi], y = w.vertices[3 * vi + 1], z = w.vertices[3 * vi + 2];
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
-      r[j][k] = m[4 * k] * x + m[4 * k + 1] * y + m[4 * k + 2] * z + m[4 * k + 3] * 1.0;
-      bad |= !isfinite(r[j][k]);
+      c[j][k] = m[4 * k] * x + m[4 * k + 1] * y + m[4 * k + 2] * z + m[4 * k + 3] * 1.0;
+      bad |= !isfinite(c[j][k]);
     }
   }
   if (bad) set_error(w.status, ERR_INPUT);
+  if ((!persp || (c[0][2] - s[5] >= 0.0 && c[1][2] - s[5] >= 0.0 && c[2][2] - s[5] >= 0.0)) &&
+      project_unclipped(w, i, c, persp))
+    return;
+  // a face that the near plane or the window clips: rings of up to SRCV
+  // vertices, indexed at run time
+  double r[SRCV][3];
+  int count = 3;
+  for (int j = 0; j < 3; ++j)
+    for (int k = 0; k < 3; ++k) r[j][k] = c[j][k];
   int was_clipped = 0;
   if (persp) {
     // perspective near clip (warp_peel/kernels.py:392-415)
