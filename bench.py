@@ -305,7 +305,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(
-            json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design only (DESIGN.md)",
+            json.dumps({"error": "this build runs on one GPU; sharding a scene or a batch over several GPUs is not built",
                         "n_gpus": args.gpus})
         )
         sys.exit(2)

@@ -125,9 +125,9 @@ int peel_read_status(PeelWorkspace *ws, int32_t *status, int64_t *metrics);
 int peel_profile(PeelWorkspace *ws, int32_t *count, float *ms, char *names, int32_t names_len);
 
 /* Launch tuning of one workspace. No counterpart in the reference, whose
- * launch shapes are fixed (its north-star autotuner picks them here); it
- * serves callers that tune a workload once and replay it many times
- * (hip_peel.tune). Outputs do not depend on these values, only the time. */
+ * launch shapes are fixed; it serves callers that tune a workload once and
+ * replay it many times (hip_peel.tune, an autotuner over these fields).
+ * Outputs do not depend on these values, only the time. */
 typedef struct PeelTuning {
   int32_t persistent_blocks_per_cu; /* k_subdivide work-queue blocks per CU, 1..max_blocks_per_cu */
   int32_t ray_lanes_target;         /* lanes in flight allowed at 64 lanes per ray (else 1 lane per ray), >= 64 */
