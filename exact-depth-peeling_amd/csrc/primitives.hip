@@ -277,7 +277,7 @@ void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan: tile sums, one-block scan of tile sums, tile scans.
+// Exclusive scan: tile sums, then tile scans offset by the sums of the tiles before them.
 constexpr int SC_THREADS = 256, SC_ITEMS = 16, SC_TILE = SC_THREADS * SC_ITEMS;
 
 
@@ -295,39 +295,24 @@ __global__ __launch_bounds__(SC_THREADS) void sc_reduce(const int *in, const int
   if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(1024) void sc_top(const int *count, int count_static, int capacity,
-                                                int *block_sums, int *total_out) {
-  __shared__ int sums[1024];
-  int n = live_count(count, count_static, capacity);
-  int nb = (n + SC_TILE - 1) / SC_TILE;
-  int t = threadIdx.x, per = (nb + 1023) / 1024;
-  int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
-  int local = 0;
-  for (int b = b0; b < b1; ++b) local += block_sums[b];
-  sums[t] = local;
-  for (int off = 1; off < 1024; off <<= 1) {
-    __syncthreads();
-    int v = t >= off ? sums[t - off] : 0;
-    __syncthreads();
-    sums[t] += v;
-  }
-  __syncthreads();
-  int running = sums[t] - local;
-  for (int b = b0; b < b1; ++b) {
-    int c = block_sums[b];
-    block_sums[b] = running;
-    running += c;
-  }
-  if (t == 1023 && total_out) *total_out = sums[1023];
-}
-
+// Each tile adds up the sums of the tiles before it (at most a few hundred,
+// two per thread and one block reduction), so the scan takes two launches;
+// the last tile writes the total.
 __global__ __launch_bounds__(SC_THREADS) void sc_apply(const int *in, int *out, const int *count,
                                                         int count_static, int capacity,
-                                                        const int *block_sums) {
+                                                        const int *block_sums, int *total_out) {
   __shared__ int lds[4];
   int n = live_count(count, count_static, capacity);
+  if (n == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && total_out) *total_out = 0;
+    return;
+  }
   int begin = blockIdx.x * SC_TILE;
   if (begin >= n) return;
+  int part = 0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += SC_THREADS) part += block_sums[b];
+  int offset;
+  block_exclusive(part, lds, offset);
   // each thread scans SC_ITEMS consecutive elements
   int first = begin + threadIdx.x * SC_ITEMS;
   int vals[SC_ITEMS];
@@ -339,7 +324,8 @@ __global__ __launch_bounds__(SC_THREADS) void sc_apply(const int *in, int *out, 
     local += vals[j];
   }
   int total;
-  int running = block_exclusive(local, lds, total) + block_sums[blockIdx.x];
+  int running = block_exclusive(local, lds, total) + offset;
+  if (threadIdx.x == 0 && total_out && begin + SC_TILE >= n) *total_out = offset + total;
 #pragma unroll
   for (int j = 0; j < SC_ITEMS; ++j) {
     int i = first + j;
@@ -354,10 +340,8 @@ void scan_exclusive(hipStream_t s, const int *in, int *out, const int *count, in
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(sc_reduce, dim3(blocks), dim3(SC_THREADS), 0, s, in, count, count_static,
                      capacity, scratch.block_sums);
-  hipLaunchKernelGGL(sc_top, dim3(1), dim3(1024), 0, s, count, count_static, capacity,
-                     scratch.block_sums, total);
   hipLaunchKernelGGL(sc_apply, dim3(blocks), dim3(SC_THREADS), 0, s, in, out, count,
-                     count_static, capacity, scratch.block_sums);
+                     count_static, capacity, scratch.block_sums, total);
 }
 
 // ---------------------------------------------------------------------------
