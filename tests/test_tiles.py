@@ -55,3 +55,43 @@ def test_tile_grid_must_be_powers_of_two():
     v, f, c, s = scenes.random_triangles(4, 0)
     with pytest.raises(ValueError):
         TiledPeeler(v, f, c, s, tiles=(3, 2))
+
+
+def test_tile_outputs_concatenate_in_window_coordinates():
+    """fragments()/triangles() join the kept tiles' CSR rings and triangles,
+    with ring offsets shifted by the vertices before them; a tile's rings are
+    already in whole-window coordinates (the map is applied per tile)."""
+    from hip_peel.tiles import Tile, _to_window
+
+    v, f, c, s = scenes.random_triangles(4, 0)
+    t = TiledPeeler(v, f, c, s, tiles=(2, 2))
+    # a ring in tile (0.5, 0, 1, 0.5) at tile-local (0, 0), (1, 0), (0, 1)
+    local = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    a = _to_window((0.5, 0.0, 1.0, 0.5), local)
+    np.testing.assert_array_equal(a, [[0.5, 0.0], [1.0, 0.0], [0.5, 0.5]])
+
+    def tile(rect, rings, sources, layers):
+        counts = [len(r) for r in rings]
+        return Tile(rect=rect, faces=np.arange(4), scale=1, info={"layers": max(layers) + 1},
+                    ring_vertices=np.concatenate(rings), ring_offsets=np.r_[0, np.cumsum(counts)],
+                    sources=np.array(sources), layers=np.array(layers, dtype=np.int32),
+                    triangles_uv=np.zeros((len(rings), 3, 2)), triangles_world=np.zeros((len(rings), 3, 3)),
+                    triangle_sources=np.array(sources), triangle_layers=np.array(layers, dtype=np.int32),
+                    device_bytes=0)
+
+    r1 = [np.array([[0.0, 0.0], [0.1, 0.0], [0.0, 0.1]]), np.array([[0.2, 0.2], [0.3, 0.2], [0.3, 0.3], [0.2, 0.3]])]
+    r2 = [np.array([[0.6, 0.6], [0.7, 0.6], [0.6, 0.7]])]
+    t.results = [tile((0, 0, 0.5, 0.5), r1, [0, 1], [0, 2]), tile((0.5, 0.5, 1, 1), r2, [3], [1])]
+    rv, off, src, lay = t.fragments()
+    np.testing.assert_array_equal(off, [0, 3, 7, 10])
+    rings = np.split(rv, off[1:-1])
+    for got, want in zip(rings, r1 + r2):
+        np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(src, [0, 1, 3])
+    np.testing.assert_array_equal(lay, [0, 2, 1])
+    assert t.layers == 3
+    uv, world, tsrc, tlay = t.triangles()
+    assert uv.shape == (3, 3, 2) and world.shape == (3, 3, 3)
+    np.testing.assert_array_equal(tsrc, [0, 1, 3])
+    t.results = []
+    assert len(t.fragments()[2]) == 0 and t.layers == 0
