@@ -78,13 +78,14 @@ def rank_mismatches(host, rings, sources, layers, parents=None):
 
 # The layered route (warp_peel/kernels.py:735-812) cuts each target by its
 # occluders in an order set by atomic pushes of the residual lists, so its
-# piece decompositions differ from run to run. On resolved random soups peeled
-# in tiles, some runs emit a few slivers one layer off along the long shared
-# edges of coplanar pieces of one parent triangle (observed: areas 8e-15 to
-# 3.7e-10 and widths up to 5e-8 of the window; the runs' other fragments, and
-# untiled runs, are exact; the cause is not identified). Such fragments are
-# admitted, and reported, below the route's own area tolerance; the event
-# route gets no allowance.
+# piece decompositions differ from run to run. On resolved random soups some
+# runs emit a few slivers one layer off along the long shared edges of
+# coplanar pieces of one parent triangle (observed over 8 runs of the 48-
+# triangle soup: 2 untiled runs with areas up to 5.5e-12, 6 tiled runs with
+# areas up to 3.7e-10 and widths up to 5e-8 of the window; every other
+# fragment exact; the cause is not identified). Such fragments are admitted,
+# and reported, below the route's own area tolerance; the event route gets no
+# allowance.
 LAYERED_SLIVER = 1e-9
 
 
