@@ -26,6 +26,7 @@ import numpy as np
 import oracle
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
+PACKAGE_DIR = Path(__file__).resolve().parents[1] / "exact-depth-peeling_amd"
 
 
 def golden_layers():
