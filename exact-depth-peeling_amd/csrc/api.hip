@@ -2,12 +2,13 @@
 // Declarations and reference citations: include/peel355.h.
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
-#include "../../include/peel355.h"
-#include "workspace.hpp"
+#include "layout.hpp"
 
 namespace peel {
 int subdivide_blocks_per_cu();
@@ -22,85 +23,33 @@ struct PeelWorkspace {
   hipGraphExec_t exec = nullptr;
   bool warmed = false;
   int device = 0;
-  std::vector<void *> allocations;
-  int64_t bytes = 0;  // device memory held by the pools
+  void *block = nullptr;  // device memory of every pool (one allocation)
+  int64_t bytes = 0;      // its size
   int cus = 1;        // compute units of the device
   int max_per_cu = 1; // resident k_subdivide blocks per CU (occupancy)
 };
 
 static thread_local std::string g_error;
 
-static int fail(int code, const std::string &message) {
+int peel::fail(int code, const std::string &message) {
   g_error = message;
   return code;
 }
 
-#define HIP_TRY(expr)                                                                  \
-  do {                                                                                 \
-    hipError_t _e = (expr);                                                            \
-    if (_e != hipSuccess)                                                              \
-      return fail(-(int)_e, std::string(#expr) + ": " + hipGetErrorString(_e));        \
-  } while (0)
 
-template <class T>
-static int alloc(PeelWorkspace *ws, T *&ptr, long count) {
-  if (count < 1) count = 1;
-  void *p = nullptr;
-  hipError_t e = hipMalloc(&p, sizeof(T) * (size_t)count);
-  if (e != hipSuccess)
-    return fail(-(int)e, "hipMalloc of " + std::to_string(sizeof(T) * (size_t)count) +
-                             " bytes: " + hipGetErrorString(e));
-  e = hipMemset(p, 0, sizeof(T) * (size_t)count);
-  if (e != hipSuccess) return fail(-(int)e, std::string("hipMemset: ") + hipGetErrorString(e));
-  ws->allocations.push_back(p);
-  ws->bytes += (int64_t)(sizeof(T) * (size_t)count);
-  ptr = (T *)p;
-  return 0;
+// Power of two at or above n, or 0 when it would exceed 2^30 (the hash tables
+// and hierarchies index with int).
+static int pow2_checked(long n) {
+  long s = 1;
+  while (s < n) s <<= 1;
+  return s > (1L << 30) ? 0 : (int)s;
 }
 
-#define ALLOC(ptr, n)                          \
-  do {                                         \
-    int _r = alloc(ws, ptr, (long)(n));        \
-    if (_r) {                                  \
-      peel_destroy(ws);                        \
-      return _r;                               \
-    }                                          \
-  } while (0)
-
-static int alloc_hierarchy(PeelWorkspace *ws, Hierarchy &h, int leaves) {
-  h.size = pow2_at_least(leaves < 1 ? 1 : leaves);
-  int r;
-  if ((r = alloc(ws, h.keys, h.size))) return r;
-  if ((r = alloc(ws, h.alt_keys, h.size))) return r;
-  if ((r = alloc(ws, h.ids, h.size))) return r;
-  if ((r = alloc(ws, h.alt_ids, h.size))) return r;
-  if ((r = alloc(ws, h.nodes, 2L * h.size))) return r;
-  if ((r = alloc(ws, h.leaves, 1))) return r;
-  return 0;
-}
-
-extern "C" {
-
-const char *peel_last_error(void) { return g_error.c_str(); }
-
-const char *peel_version(void) { return "peel355 0.1 gfx950"; }
-
-int peel_destroy(PeelWorkspace *ws) {
-  if (!ws) return 0;
-  if (ws->exec) hipGraphExecDestroy(ws->exec);
-  if (ws->graph) hipGraphDestroy(ws->graph);
-  if (ws->stream) hipStreamSynchronize(ws->stream);
-  for (void *p : ws->allocations) hipFree(p);
-  if (ws->stream) hipStreamDestroy(ws->stream);
-  delete ws;
-  return 0;
-}
-
-int peel_create(const PeelDesc *d, PeelWorkspace **out) {
-  if (!d || !out) return fail(1, "null argument");
-  *out = nullptr;
+// Validate the descriptor and set every scalar of the workspace layout.
+// Returns 0 or a positive argument error (message in g_error).
+int peel::plan(Ws &w, const PeelDesc *d) {
   if (d->num_vertices < 0 || d->num_faces < 0 || d->num_vertices > (1L << 30) ||
-      d->num_faces > (1L << 27))
+      d->num_faces > (1L << 26))
     return fail(2, "vertex or face count out of range");
   if ((d->num_vertices && !d->vertices) || (d->num_faces && !d->faces) || !d->camera || !d->settings)
     return fail(3, "null device array");
@@ -113,9 +62,6 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   if (d->route != PEEL_ROUTE_EVENTS && d->route != PEEL_ROUTE_LAYERED) return fail(7, "unknown route");
   if (d->route == PEEL_ROUTE_LAYERED && (d->workers < 1 || d->pieces < 1))
     return fail(4, "Capacities must be positive");
-  PeelWorkspace *ws = new PeelWorkspace();
-  HIP_TRY(hipGetDevice(&ws->device));
-  Ws &w = ws->w;
   std::memset(&w, 0, sizeof(Ws));
   w.vertices = d->vertices;
   w.faces = d->faces;
@@ -130,151 +76,213 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
   w.share_regions = (d->flags & PEEL_SHARE_REGIONS) != 0;
   w.propagate_ranks = (d->flags & PEEL_PROPAGATE_RANKS) != 0;
   w.merge_regions = (d->flags & PEEL_MERGE_REGIONS) != 0;
+  w.filter32 = (d->flags & PEEL_FILTER_FP32) != 0;
   w.route = d->route;
   const bool layered = d->route == PEEL_ROUTE_LAYERED;
   w.workers = layered ? d->workers : 1;
   w.pieces = layered ? d->pieces : 1;
-  int nf1 = w.nf > 0 ? w.nf : 1, nv1 = w.nv > 0 ? w.nv : 1;
-  ALLOC(w.status, 16);
-  ALLOC(w.metrics, M_COUNT);
-  ALLOC(w.control, C_COUNT);
-  ALLOC(w.inverse, 16);
-  ALLOC(w.canonical, nv1);
-  w.vhash_size = pow2_at_least(2 * nv1);
-  ALLOC(w.vhash, w.vhash_size);
-  w.ehash_size = pow2_at_least(6 * nf1);
-  ALLOC(w.ekeys, w.ehash_size);
-  ALLOC(w.ecount, w.ehash_size);
-  ALLOC(w.efirst, w.ehash_size);
-  ALLOC(w.esecond, w.ehash_size);
-  ALLOC(w.masks, 3L * nf1);
-  ALLOC(w.src_xy, (long)SRCV * nf1);
-  ALLOC(w.src_n, nf1);
-  ALLOC(w.scores, nf1);
-  ALLOC(w.facing, nf1);
-  ALLOC(w.clipped, nf1);
-  ALLOC(w.src_boxes, nf1);
-  w.nseg = 8 * nf1;
-  ALLOC(w.seg_a, w.nseg);
-  ALLOC(w.seg_b, w.nseg);
-  ALLOC(w.seg_near, 2L * w.nseg);
-  ALLOC(w.seg_rep, w.nseg);
-  w.shash_size = pow2_at_least(2 * w.nseg);
-  ALLOC(w.shash, w.shash_size);
-  ALLOC(w.seg_boxes, w.nseg);
-  ALLOC(w.seg_flag, w.nseg);
-  ALLOC(w.seg_off, w.nseg);
-  ALLOC(w.cseg_a, w.nseg);
-  ALLOC(w.cseg_b, w.nseg);
-  ALLOC(w.cnear, 2L * w.nseg);
-  int r;
-  if ((r = alloc_hierarchy(ws, w.source_tree, nf1)) || (r = alloc_hierarchy(ws, w.contour_tree, w.nseg))) {
-    peel_destroy(ws);
-    return r;
-  }
-  ALLOC(w.leaf_xy, 8L * w.source_tree.size);
-  ALLOC(w.leaf_n, w.source_tree.size);
-  ALLOC(w.leaf_score, w.source_tree.size);
-  w.filter32 = (d->flags & PEEL_FILTER_FP32) != 0;
-  ALLOC(w.leaf_xy32, w.filter32 ? 8L * w.source_tree.size : 1);
-  ALLOC(w.leaf_tol, w.filter32 ? w.source_tree.size : 1);
-  long F = layered ? 1 : w.cap.frontier, R = w.cap.regions;
-  // layered route: ping-pong residual pools sized by the frontier capacity,
-  // per-worker piece pools, per-layer statistics
-  long RES = layered ? w.cap.frontier : 1, NF = layered ? nf1 : 1;
-  long POOL = layered ? 3L * (w.pieces + 3) * w.workers : 1;
-  ALLOC(w.ranks, NF);
-  ALLOC(w.contour_boxes, NF);
-  for (int par = 0; par < 2; ++par) {
-    ALLOC(w.res_xy[par], RES * MAXV);
-    ALLOC(w.res_n[par], RES);
-    ALLOC(w.res_src[par], RES);
-    ALLOC(w.res_run[par], RES);
-    ALLOC(w.res_link[par], RES);
-    ALLOC(w.res_box[par], RES);
-    ALLOC(w.heads[par], NF);
-  }
-  ALLOC(w.pool_xy, POOL * MAXV);
-  ALLOC(w.pool_n, POOL);
-  ALLOC(w.stats, 3L * w.cap.layers);
-  ALLOC(w.fr_xy, F * MAXV);
-  ALLOC(w.fr_n, F);
-  ALLOC(w.fr_src, F);
-  ALLOC(w.fr_node, F);
-  ALLOC(w.fr_gen, F);
-  ALLOC(w.fr_ready, F);
-  ALLOC(w.out_xy, R * MAXV);
-  ALLOC(w.out_n, R);
-  ALLOC(w.out_src, R);
-  ALLOC(w.out_layer, R);
-  ALLOC(w.samples, R);
-  ALLOC(w.reps, R);
-  w.qhash_size = pow2_at_least(2 * w.cap.regions);
-  ALLOC(w.qhash, w.qhash_size);
-  ALLOC(w.groups, R);
-  ALLOC(w.group_counts, R);
-  ALLOC(w.group_offsets, R);
-  ALLOC(w.group_fill, R);
-  ALLOC(w.ray_keys, R);
-  ALLOC(w.ray_keys_alt, R);
-  ALLOC(w.ray_ids, R);
-  ALLOC(w.ray_ids_alt, R);
-  ALLOC(w.medium, R);
-  ALLOC(w.heavy, R);
-  ALLOC(w.beam_box, 4L * BEAM_CELLS);
-  ALLOC(w.beam_leader, BEAM_CELLS);
-  ALLOC(w.beam_valid, BEAM_CELLS);
-  ALLOC(w.beam_jobs, BEAM_CELLS);
-  long H = w.cap.hits;
-  ALLOC(w.hit_depth, H);
-  ALLOC(w.hit_src, H);
-  ALLOC(w.hit_group, H);
-  ALLOC(w.hkeys, H);
-  ALLOC(w.hkeys_alt, H);
-  ALLOC(w.hids, H);
-  ALLOC(w.hids_alt, H);
-  long BV = w.cap.vertices, T = w.cap.triangles;
-  ALLOC(w.owners, BV);
-  ALLOC(w.starts, BV);
-  ALLOC(w.ends, BV);
-  ALLOC(w.partners, BV);
-  ALLOC(w.parents, R);
-  ALLOC(w.choices, R);
-  long MR = w.merge_regions ? R : 1;
-  ALLOC(w.merge_xy, MR * MAXV);
-  ALLOC(w.merge_n, MR);
-  ALLOC(w.merge_src, MR);
-  ALLOC(w.merge_layer, MR);
-  w.ehash2_size = pow2_at_least(2 * (int)std::min<long>(BV, 1L << 29));
-  ALLOC(w.eslots, w.ehash2_size);
-  ALLOC(w.vcounts, R);
-  ALLOC(w.voffsets, R);
-  ALLOC(w.bverts, BV);
-  ALLOC(w.blayers, BV);
-  ALLOC(w.tasks, BV);
-  ALLOC(w.crowded, BV);
-  w.phash_size = pow2_at_least(2 * (int)std::min<long>(BV, 1L << 29));
-  ALLOC(w.phash, w.phash_size);
+  const long nf1 = w.nf > 0 ? w.nf : 1, nv1 = w.nv > 0 ? w.nv : 1;
+  const long BV = w.cap.vertices;
+  w.nseg = (int)(8 * nf1);
+  w.vhash_size = pow2_checked(2 * nv1);
+  w.ehash_size = pow2_checked(6 * nf1);
+  w.shash_size = pow2_checked(2L * w.nseg);
+  w.qhash_size = pow2_checked(2L * w.cap.regions);
+  w.ehash2_size = pow2_checked(2 * std::min<long>(BV, 1L << 29));
+  w.phash_size = pow2_checked(2 * std::min<long>(BV, 1L << 29));
+  w.source_tree.size = pow2_checked(nf1);
+  w.contour_tree.size = pow2_checked(w.nseg);
+  for (int s : {w.vhash_size, w.ehash_size, w.shash_size, w.qhash_size, w.ehash2_size, w.phash_size,
+                w.source_tree.size, w.contour_tree.size})
+    if (s == 0) return fail(2, "mesh or capacities too large: a hash table or hierarchy would exceed 2^30 slots");
   w.cell_capacity = 4 * BV;
-  ALLOC(w.cell_heads, w.cell_capacity);
-  ALLOC(w.cell_next, BV);
-  ALLOC(w.tri_uv, T * 6);
-  ALLOC(w.tri_world, T * 9);
-  ALLOC(w.tri_src, T);
-  ALLOC(w.tri_layer, T);
-  long sort_cap = H;
-  sort_cap = std::max<long>(sort_cap, w.source_tree.size);
-  sort_cap = std::max<long>(sort_cap, w.contour_tree.size);
-  sort_cap = std::max<long>(sort_cap, R);
+  long sort_cap = std::max<long>({(long)w.cap.hits, (long)w.source_tree.size, (long)w.contour_tree.size,
+                                  (long)w.cap.regions, layered ? (long)w.cap.frontier : 1L});
   w.sort.max_blocks = (int)((sort_cap + RS_TILE - 1) / RS_TILE);
-  ALLOC(w.sort.counts, RS_MAX_SWEEPS * 256);
-  ALLOC(w.sort.tickets, RS_MAX_SWEEPS);
-  ALLOC(w.sort.tiles, (long)RS_MAX_SWEEPS * w.sort.max_blocks * 256);
-  ALLOC(w.sort.flags, RS_MAX_SWEEPS);
-  w.sort.status = w.status;
-  long scan_cap = std::max<long>(std::max<long>(R, BV), w.nseg);
+  long scan_cap = std::max<long>(std::max<long>(w.cap.regions, BV), w.nseg);
   w.scan.max_blocks = (int)((scan_cap + 4095) / 4096) + 1;
-  ALLOC(w.scan.block_sums, w.scan.max_blocks);
+  w.ray_lanes_target = 524288;
+  w.heavy_lanes = 65536;
+  w.long_edge_cells = 48;
+  return 0;
+}
+
+static void carve_hierarchy(Carver &c, Hierarchy &h) {
+  c.take(h.keys, h.size);
+  c.take(h.alt_keys, h.size);
+  c.take(h.ids, h.size);
+  c.take(h.alt_ids, h.size);
+  c.take(h.nodes, 2L * h.size);
+  c.take(h.leaves, 1);
+}
+
+// Every pool of a planned workspace. `outputs` false leaves the caller-owned
+// outputs (polygons, triangles, status, metrics) to the caller.
+void peel::carve(Ws &w, Carver &c, bool outputs) {
+  const bool layered = w.route == PEEL_ROUTE_LAYERED;
+  const long nf1 = w.nf > 0 ? w.nf : 1, nv1 = w.nv > 0 ? w.nv : 1;
+  const long R = w.cap.regions, H = w.cap.hits, BV = w.cap.vertices, T = w.cap.triangles;
+  if (outputs) {
+    c.take(w.status, 16);
+    c.take(w.metrics, M_COUNT);
+    c.take(w.out_xy, R * MAXV);
+    c.take(w.out_n, R);
+    c.take(w.out_src, R);
+    c.take(w.out_layer, R);
+    c.take(w.tri_uv, T * 6);
+    c.take(w.tri_world, T * 9);
+    c.take(w.tri_src, T);
+    c.take(w.tri_layer, T);
+  }
+  c.take(w.control, C_COUNT);
+  c.take(w.inverse, 16);
+  c.take(w.canonical, nv1);
+  c.take(w.vhash, w.vhash_size);
+  c.take(w.ekeys, w.ehash_size);
+  c.take(w.ecount, w.ehash_size);
+  c.take(w.efirst, w.ehash_size);
+  c.take(w.esecond, w.ehash_size);
+  c.take(w.masks, 3 * nf1);
+  c.take(w.src_xy, SRCV * nf1);
+  c.take(w.src_n, nf1);
+  c.take(w.scores, nf1);
+  c.take(w.facing, nf1);
+  c.take(w.clipped, nf1);
+  c.take(w.src_boxes, nf1);
+  c.take(w.seg_a, w.nseg);
+  c.take(w.seg_b, w.nseg);
+  c.take(w.seg_near, 2L * w.nseg);
+  c.take(w.seg_rep, w.nseg);
+  c.take(w.shash, w.shash_size);
+  c.take(w.seg_boxes, w.nseg);
+  c.take(w.seg_flag, w.nseg);
+  c.take(w.seg_off, w.nseg);
+  c.take(w.cseg_a, w.nseg);
+  c.take(w.cseg_b, w.nseg);
+  c.take(w.cnear, 2L * w.nseg);
+  carve_hierarchy(c, w.source_tree);
+  carve_hierarchy(c, w.contour_tree);
+  c.take(w.leaf_xy, 8L * w.source_tree.size);
+  c.take(w.leaf_n, w.source_tree.size);
+  c.take(w.leaf_score, w.source_tree.size);
+  c.take(w.leaf_xy32, w.filter32 ? 8L * w.source_tree.size : 1);
+  c.take(w.leaf_tol, w.filter32 ? w.source_tree.size : 1);
+  // layered route: residual pools sized by the frontier capacity, per-worker
+  // piece pools, per-layer statistics
+  const long RES = layered ? w.cap.frontier : 1, NF = layered ? nf1 : 1;
+  const long POOL = layered ? 3L * (w.pieces + 3) * w.workers : 1;
+  c.take(w.ranks, NF);
+  c.take(w.contour_boxes, NF);
+  for (int par = 0; par < 2; ++par) {
+    c.take(w.res_xy[par], RES * MAXV);
+    c.take(w.res_n[par], RES);
+    c.take(w.res_src[par], RES);
+    c.take(w.res_run[par], RES);
+    c.take(w.res_link[par], RES);
+    c.take(w.res_box[par], RES);
+    c.take(w.heads[par], NF);
+  }
+  c.take(w.pool_xy, POOL * MAXV);
+  c.take(w.pool_n, POOL);
+  c.take(w.stats, 3L * w.cap.layers);
+  const long F = layered ? 1 : w.cap.frontier;
+  c.take(w.fr_xy, F * MAXV);
+  c.take(w.fr_n, F);
+  c.take(w.fr_src, F);
+  c.take(w.fr_node, F);
+  c.take(w.fr_gen, F);
+  c.take(w.fr_ready, F);
+  c.take(w.samples, R);
+  c.take(w.reps, R);
+  c.take(w.qhash, w.qhash_size);
+  c.take(w.groups, R);
+  c.take(w.group_counts, R);
+  c.take(w.group_offsets, R);
+  c.take(w.group_fill, R);
+  c.take(w.ray_keys, R);
+  c.take(w.ray_keys_alt, R);
+  c.take(w.ray_ids, R);
+  c.take(w.ray_ids_alt, R);
+  c.take(w.medium, R);
+  c.take(w.heavy, R);
+  c.take(w.beam_box, 4L * BEAM_CELLS);
+  c.take(w.beam_leader, BEAM_CELLS);
+  c.take(w.beam_valid, BEAM_CELLS);
+  c.take(w.beam_jobs, BEAM_CELLS);
+  c.take(w.hit_depth, H);
+  c.take(w.hit_src, H);
+  c.take(w.hit_group, H);
+  c.take(w.hkeys, H);
+  c.take(w.hkeys_alt, H);
+  c.take(w.hids, H);
+  c.take(w.hids_alt, H);
+  c.take(w.owners, BV);
+  c.take(w.starts, BV);
+  c.take(w.ends, BV);
+  c.take(w.partners, BV);
+  c.take(w.parents, R);
+  c.take(w.choices, R);
+  const long MR = w.merge_regions ? R : 1;
+  c.take(w.merge_xy, MR * MAXV);
+  c.take(w.merge_n, MR);
+  c.take(w.merge_src, MR);
+  c.take(w.merge_layer, MR);
+  c.take(w.eslots, w.ehash2_size);
+  c.take(w.vcounts, R);
+  c.take(w.voffsets, R);
+  c.take(w.bverts, BV);
+  c.take(w.blayers, BV);
+  c.take(w.tasks, BV);
+  c.take(w.crowded, BV);
+  c.take(w.phash, w.phash_size);
+  c.take(w.cell_heads, w.cell_capacity);
+  c.take(w.cell_next, BV);
+  c.take(w.sort.counts, RS_MAX_SWEEPS * 256);
+  c.take(w.sort.tickets, RS_MAX_SWEEPS);
+  c.take(w.sort.tiles, (long)RS_MAX_SWEEPS * w.sort.max_blocks * 256);
+  c.take(w.sort.flags, RS_MAX_SWEEPS);
+  c.take(w.scan.block_sums, w.scan.max_blocks);
+  w.sort.status = w.status;
+}
+
+extern "C" {
+
+const char *peel_last_error(void) { return g_error.c_str(); }
+
+const char *peel_version(void) { return "peel355 0.2 gfx950"; }
+
+int peel_destroy(PeelWorkspace *ws) {
+  if (!ws) return 0;
+  if (ws->exec) hipGraphExecDestroy(ws->exec);
+  if (ws->graph) hipGraphDestroy(ws->graph);
+  if (ws->stream) hipStreamSynchronize(ws->stream);
+  if (ws->block) hipFree(ws->block);
+  if (ws->stream) hipStreamDestroy(ws->stream);
+  delete ws;
+  return 0;
+}
+
+}  // extern "C"
+
+// The body of peel_create on a fresh workspace; the caller destroys it on
+// failure, which releases whatever was already created.
+static int create_into(PeelWorkspace *ws, const PeelDesc *d) {
+  HIP_TRY(hipGetDevice(&ws->device));
+  Ws &w = ws->w;
+  int r = plan(w, d);
+  if (r) return r;
+  Carver measure;
+  carve(w, measure, true);
+  void *block = nullptr;
+  hipError_t e = hipMalloc(&block, measure.used);
+  if (e != hipSuccess)
+    return fail(-(int)e, "hipMalloc of " + std::to_string(measure.used) + " bytes: " + hipGetErrorString(e));
+  ws->block = block;
+  ws->bytes = (int64_t)measure.used;
+  HIP_TRY(hipMemset(block, 0, measure.used));
+  Carver pools;
+  pools.base = static_cast<char *>(block);
+  carve(w, pools, true);
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ws->device));
   int per_cu = subdivide_blocks_per_cu();
@@ -287,24 +295,27 @@ int peel_create(const PeelDesc *d, PeelWorkspace **out) {
     int v = std::atoi(env);
     if (v > 0) w.persistent_blocks = std::min(v, ws->cus * ws->max_per_cu);
   }
-  w.ray_lanes_target = 524288;
-  w.heavy_lanes = 65536;
-  w.long_edge_cells = 48;
-  hipError_t e = hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) {
+  HIP_TRY(hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking));
+  HIP_TRY(hipDeviceSynchronize());
+  return 0;
+}
+
+extern "C" int peel_create(const PeelDesc *d, PeelWorkspace **out) {
+  if (!d || !out) return fail(1, "null argument");
+  *out = nullptr;
+  PeelWorkspace *ws = new PeelWorkspace();
+  int r = create_into(ws, d);
+  if (r) {
+    std::string message = g_error;
     peel_destroy(ws);
-    return fail(-(int)e, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-  }
-  e = hipDeviceSynchronize();
-  if (e != hipSuccess) {
-    peel_destroy(ws);
-    return fail(-(int)e, std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
+    g_error = message;
+    return r;
   }
   *out = ws;
   return 0;
 }
 
-static void enqueue_all(const Ws &w, hipStream_t s, Timeline *t) {
+void peel::enqueue_all(const Ws &w, hipStream_t s, Timeline *t) {
   enqueue_prepare(w, s, t);
   if (w.route == PEEL_ROUTE_LAYERED) {
     enqueue_layered(w, s, t);
@@ -314,6 +325,8 @@ static void enqueue_all(const Ws &w, hipStream_t s, Timeline *t) {
   }
   enqueue_mesh(w, s, t);
 }
+
+extern "C" {
 
 int peel_enqueue(PeelWorkspace *ws) {
   if (!ws) return fail(1, "null workspace");

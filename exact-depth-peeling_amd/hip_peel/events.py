@@ -112,13 +112,12 @@ class EventPeeler(_Workspace):
         )
         self._open(vertices, faces, camera, settings, caps, layers, flags)
 
-    def read_status(self):
-        status, metrics = self._read()
+    def _decode(self, status, metrics):
         if status[9] & 1024:
             raise RuntimeError("GPU extraction failed: shared ray hit capacity")
         if status[9] & 2048:
             raise RuntimeError("GPU extraction failed: subdivision round capacity")
-        info = super().read_status()
+        info = super()._decode(status, metrics)
         for name in ("peeling_passes", "max_hidden_pieces"):
             info.pop(name)
         info.update(zip(METRIC_NAMES, map(int, metrics)))

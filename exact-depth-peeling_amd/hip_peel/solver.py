@@ -274,8 +274,13 @@ class _Workspace:
         return status, metrics
 
     def read_status(self):
-        """Synchronise and decode the device status (solver.py:488-519)."""
-        values, self._last_metrics = self._read()
+        """Synchronise once, copy status and metrics once, and decode them
+        (solver.py:488-519)."""
+        values, metrics = self._read()
+        self._last_metrics = metrics
+        return self._decode(values, metrics)
+
+    def _decode(self, values, metrics):
         if values[9]:
             raise RuntimeError(
                 "GPU extraction failed: " + ", ".join(v for b, v in ERROR_LABELS.items() if values[9] & b)

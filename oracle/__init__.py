@@ -264,3 +264,42 @@ def cpu_baseline(host, seconds=20.0, threads=1, workload=""):
         "polygons": result.counts["polygons"],
         "layers": result.counts["layers"],
     }
+
+
+LAYERED_COUNT_FIELDS = ("polygons", "layers", "error", "depth_ties", "passes", "peak_residual", "max_hidden",
+                        "whole_sources", "residual_sources", "candidate_tests")
+
+
+class LayeredResult:
+    def __init__(self, rings, sources, layers, counts, stats):
+        self.rings, self.sources, self.layers = rings, sources, layers
+        self.counts, self.stats = counts, stats
+
+
+def cpu_layered_peel(vertices, faces, camera, settings, layers=0, layer_capacity=128, pieces=None, residual=None,
+                     cap_poly=None, order_seed=0):
+    """CPU restatement of the layered (DevicePeeler) route in the device's
+    candidate order; ``order_seed`` != 0 shuffles each pass's residual order."""
+    vertices = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+    faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+    camera = np.ascontiguousarray(camera, dtype=np.float64).reshape(4, 4)
+    settings = np.ascontiguousarray(settings, dtype=np.float64).reshape(6)
+    n = max(16, len(faces))
+    pieces = pieces or (512 if n <= 8192 else 2048)
+    residual = residual or 4 * n
+    cap_poly = cap_poly or 16 * n
+    poly_xy = np.zeros((cap_poly, 32, 2))
+    poly_n = np.zeros(cap_poly, np.int32)
+    poly_src = np.zeros(cap_poly, np.int32)
+    poly_layer = np.zeros(cap_poly, np.int32)
+    counts = np.zeros(len(LAYERED_COUNT_FIELDS), np.int64)
+    stats = np.zeros((layer_capacity, 3), np.int64)
+    f = lib().oracle_cpu_layered_peel
+    f.restype = ctypes.c_int
+    f(_p(vertices), len(vertices), _p(faces), len(faces), _p(camera), _p(settings), int(layers),
+      int(layer_capacity), int(pieces), int(residual), ctypes.c_uint(order_seed), _p(poly_xy), _p(poly_n),
+      _p(poly_src), _p(poly_layer), int(cap_poly), _p(counts), _p(stats), int(layer_capacity))
+    c = dict(zip(LAYERED_COUNT_FIELDS, map(int, counts)))
+    P = min(c["polygons"], cap_poly)
+    rings = [poly_xy[i, : poly_n[i]].copy() for i in range(P)]
+    return LayeredResult(rings, poly_src[:P].copy(), poly_layer[:P].copy(), c, stats[: c["passes"]].copy())
