@@ -267,10 +267,10 @@ int peel_destroy(PeelWorkspace *ws) {
 // The body of peel_create on a fresh workspace; the caller destroys it on
 // failure, which releases whatever was already created.
 static int create_into(PeelWorkspace *ws, const PeelDesc *d) {
-  HIP_TRY(hipGetDevice(&ws->device));
   Ws &w = ws->w;
   int r = plan(w, d);
   if (r) return r;
+  HIP_TRY(hipGetDevice(&ws->device));
   Carver measure;
   carve(w, measure, true);
   void *block = nullptr;
