@@ -63,6 +63,26 @@ class PeelBuffers(ctypes.Structure):
     ]
 
 
+class PeelOutputs(ctypes.Structure):
+    """Caller-owned outputs of peel_extract_static (include/peel355.h)."""
+
+    _fields_ = [
+        ("output_xy", ctypes.c_void_p),
+        ("output_n", ctypes.c_void_p),
+        ("output_sources", ctypes.c_void_p),
+        ("output_layers", ctypes.c_void_p),
+        ("triangles_uv", ctypes.c_void_p),
+        ("triangles_world", ctypes.c_void_p),
+        ("triangle_sources", ctypes.c_void_p),
+        ("triangle_layers", ctypes.c_void_p),
+        ("status", ctypes.c_void_p),
+        ("metrics", ctypes.c_void_p),
+    ]
+
+
+STATIC_SCRATCH_BYTES = 256 << 20
+
+
 class PeelTuning(ctypes.Structure):
     _fields_ = [
         ("persistent_blocks_per_cu", ctypes.c_int32),
@@ -97,6 +117,7 @@ EXPORTS = (
     "pool_mlp_forward",
     "pool_mlp_gradient",
     "pool_mlp_adam",
+    "peel_extract_static",
 )
 
 _lib = None
@@ -144,6 +165,8 @@ def load():
     lib.pool_mlp_gradient.restype = ctypes.c_int
     lib.pool_mlp_adam.argtypes = [vp, vp, vp, vp, i32, f32, f32, f32, f32, f32, f32, vp]
     lib.pool_mlp_adam.restype = ctypes.c_int
+    lib.peel_extract_static.argtypes = [ctypes.POINTER(PeelDesc), ctypes.POINTER(PeelOutputs), vp]
+    lib.peel_extract_static.restype = ctypes.c_int
     lib.peel_last_error.restype = ctypes.c_char_p
     lib.peel_version.restype = ctypes.c_char_p
     _lib = lib

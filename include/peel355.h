@@ -186,6 +186,37 @@ int pool_mlp_gradient(const float *params, int32_t inputs, int32_t outputs, cons
 int pool_mlp_adam(float *params, const float *grad, float *m, float *v, int32_t count, float lr, float beta1,
                   float beta2, float eps, float bias1, float bias2, void *stream);
 
+/* One extraction that allocates nothing. No counterpart in the reference; it
+ * serves callers that cannot allocate device memory (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Inputs and
+ * capacities as for peel_create; the caller owns the outputs, sized by
+ * desc->regions and desc->triangles: output_xy [32][regions][2] vertex-major,
+ * output_n/sources/layers [regions], triangles_uv [triangles][3][2],
+ * triangles_world [triangles][3][3], triangle_sources/layers [triangles],
+ * status [16] and metrics [PEEL_METRICS_COUNT], with the meaning of
+ * PeelBuffers. Every other pool is carved from a device array of
+ * PEEL_STATIC_SCRATCH_BYTES that each GPU holds for the life of the process;
+ * a descriptor needing more returns 2 with the needed size in
+ * peel_last_error(). The pipeline is enqueued on `stream` (hipStream_t, NULL =
+ * the null stream) without synchronising; read the status after the stream.
+ * Calls on one GPU share its array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be ordered by the caller. */
+#define PEEL_STATIC_SCRATCH_BYTES (256ull << 20)
+#define PEEL_METRICS_COUNT 24
+typedef struct PeelOutputs {
+  double *output_xy;
+  int32_t *output_n;
+  int32_t *output_sources;
+  int32_t *output_layers;
+  double *triangles_uv;
+  double *triangles_world;
+  int32_t *triangle_sources;
+  int32_t *triangle_layers;
+  int32_t *status;
+  int64_t *metrics;
+} PeelOutputs;
+int peel_extract_static(const PeelDesc *desc, const PeelOutputs *outputs, void *stream);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *peel_last_error(void);
 

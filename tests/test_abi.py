@@ -87,7 +87,7 @@ def _header_fields(struct):
     return re.findall(r"([A-Za-z_][A-Za-z_0-9]*)\s*;", body)
 
 
-@pytest.mark.parametrize("struct", ["PeelDesc", "PeelBuffers"])
+@pytest.mark.parametrize("struct", ["PeelDesc", "PeelBuffers", "PeelOutputs"])
 def test_ctypes_structs_follow_the_header(struct):
     """The ctypes binding lists the header's fields in the header's order."""
     from hip_peel import _lib
