@@ -96,6 +96,13 @@ int peel::plan(Ws &w, const PeelDesc *d) {
                 w.source_tree.size, w.contour_tree.size})
     if (s == 0) return fail(2, "mesh or capacities too large: a hash table or hierarchy would exceed 2^30 slots");
   w.cell_capacity = 4 * BV;
+  auto bits_for = [](long n) {
+    int b = 1;
+    while ((1L << b) < n) ++b;
+    return b;
+  };
+  w.res_parent_bits = bits_for(layered ? w.cap.frontier : 1);
+  w.res_key_bits = w.res_parent_bits + bits_for(nf1);
   long sort_cap = std::max<long>({(long)w.cap.hits, (long)w.source_tree.size, (long)w.contour_tree.size,
                                   (long)w.cap.regions, layered ? (long)w.cap.frontier : 1L});
   w.sort.max_blocks = (int)((sort_cap + RS_TILE - 1) / RS_TILE);
@@ -177,11 +184,14 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
     c.take(w.res_xy[par], RES * MAXV);
     c.take(w.res_n[par], RES);
     c.take(w.res_src[par], RES);
-    c.take(w.res_run[par], RES);
-    c.take(w.res_link[par], RES);
     c.take(w.res_box[par], RES);
-    c.take(w.heads[par], NF);
   }
+  c.take(w.res_keys, RES);
+  c.take(w.res_keys_alt, RES);
+  c.take(w.res_ids, RES);
+  c.take(w.res_ids_alt, RES);
+  c.take(w.run_first, NF);
+  c.take(w.run_last, NF);
   c.take(w.pool_xy, POOL * MAXV);
   c.take(w.pool_n, POOL);
   c.take(w.stats, 3L * w.cap.layers);

@@ -20,12 +20,19 @@
 // parts (or the whole piece, if nothing stays visible) the next residual.
 //
 // MI355X layout instead of the reference's per-layer hierarchies: one static
-// source hierarchy answers every candidate query. Residual pieces are stored
-// in runs (consecutive slots of one source) chained per source, so a query
-// that reaches a source's leaf walks that source's live pieces; nothing is
-// rebuilt or sorted between passes. The per-worker piece pools are
-// vertex-major with the worker index fastest, so the lanes of a wave touch
-// adjacent addresses when they walk their pieces in step.
+// source hierarchy answers every candidate query. After classification and
+// after every pass the residual pieces are ordered by (source, parent slot)
+// with a stable radix sort, so each source's live pieces are one run of
+// consecutive slots and a query that reaches a source's leaf walks that run;
+// the order is a function of the input, so every run cuts in the same order.
+// The per-worker piece pools are vertex-major with the worker index fastest,
+// so the lanes of a wave touch adjacent addresses when they walk their pieces
+// in step.
+//
+// Every ring this route constructs is made strictly convex (convexify_ring):
+// `separated` and the edge-by-edge clipping of `cut` rely on each edge line
+// having the whole ring on its left, which a slightly reflex constructed
+// vertex breaks.
 #include <algorithm>
 
 #include "workspace.hpp"
@@ -45,6 +52,43 @@ __device__ __forceinline__ RingRef source_ring(const Ws &w, int i) {
 }
 __device__ __forceinline__ RingRef residual_ring(const Ws &w, int par, int k) {
   return RingRef{w.res_xy[par] + k, (long)w.cap.frontier};
+}
+
+// Sort key of an appended residual piece: its source, then the slot of the
+// target it was hidden under (pieces of one target are appended in order).
+__device__ __forceinline__ uint64_t residual_key(const Ws &w, int source, int parent) {
+  return ((uint64_t)(uint32_t)source << w.res_parent_bits) | (uint64_t)(uint32_t)parent;
+}
+
+// Strict convexity of a constructed ring (not in the reference): a vertex
+// whose turn is not left by more than TURN_TOL times the L1 lengths of its two
+// chords is dropped, until every turn is. The bound exceeds the rounding error
+// of the turn, so every kept turn is left in exact arithmetic; a dropped
+// vertex lies within about TURN_TOL of its neighbours' chord. Two constructed
+// points ~1e-12 apart can otherwise form a slightly reflex vertex whose short
+// edge has a rotated supporting line, and clipping by that line misplaces a
+// strip up to ~1e-9 wide on the wrong layer.
+constexpr double TURN_TOL = 1.0e-12;
+__device__ int convexify_ring(RingRef p, int n) {
+  if (n > MAXV) return n;
+  bool changed = true;
+  while (changed && n >= 3) {
+    changed = false;
+    for (int j = 0; j < n && n >= 3;) {
+      double2 a = p[(j + n - 1) % n], b = p[j], c = p[(j + 1) % n];
+      double ux = b.x - a.x, uy = b.y - a.y, wx = c.x - a.x, wy = c.y - a.y;
+      double turn = ux * wy - uy * wx;
+      double scale = fabs(ux) + fabs(uy) + fabs(wx) + fabs(wy);
+      if (turn <= TURN_TOL * scale) {
+        for (int k = j; k + 1 < n; ++k) p.at(k) = p[k + 1];
+        n -= 1;
+        changed = true;
+      } else {
+        ++j;
+      }
+    }
+  }
+  return n < 3 ? 0 : n;
 }
 
 // Worker pools: q = 0, 1 alternate as visible/temporary, q = 2 holds hidden
@@ -88,8 +132,8 @@ __device__ __forceinline__ bool contour_edge(const Ws &w, int i, int j) {
 __global__ void k_layered_reset(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   for (int j = i; j < w.nf; j += gridDim.x * blockDim.x) {
-    w.heads[0][j] = -1;
-    w.heads[1][j] = -1;
+    w.run_first[j] = -1;
+    w.run_last[j] = -1;
   }
   for (int j = i; j < 3 * w.cap.layers; j += gridDim.x * blockDim.x) w.stats[j] = 0;
   if (i == 0) w.control[C_LOOP] = 0;
@@ -143,12 +187,6 @@ __device__ void save_polygon(const Ws &w, RingRef p, int n, int source, int laye
   } else {
     set_error(w.status, ERR_OUTPUT);
   }
-}
-
-// Chain a run of `count` pieces starting at `slot` to its source in pool `par`.
-__device__ void push_residual(const Ws &w, int par, int slot, int count, int source) {
-  w.res_run[par][slot] = count;
-  w.res_link[par][slot] = atomicExch(w.heads[par] + source, slot);
 }
 
 // classify (kernels.py:597-683)
@@ -224,11 +262,14 @@ __global__ __launch_bounds__(256) void k_classify(Ws w) {
   if (rank < 0) {
     int slot = atomicAdd(w.status + ST_CUR, 1);
     if (slot < w.cap.frontier) {
-      copy_ring(p, n, residual_ring(w, 0, slot));
-      w.res_n[0][slot] = n;
-      w.res_src[0][slot] = i;
-      w.res_box[0][slot] = w.src_boxes[i];
-      push_residual(w, 0, slot, 1, i);
+      RingRef q = residual_ring(w, 1, slot);
+      copy_ring(p, n, q);
+      int qn = convexify_ring(q, n);
+      w.res_n[1][slot] = qn;
+      w.res_src[1][slot] = i;
+      w.res_box[1][slot] = ring_box(q, qn, w.scores[i]);
+      w.res_keys[slot] = residual_key(w, i, 0);
+      w.res_ids[slot] = slot;
     } else {
       set_error(w.status, ERR_RESIDUAL);
     }
@@ -243,7 +284,9 @@ __global__ __launch_bounds__(256) void k_classify(Ws w) {
 // start_loop (kernels.py:686-689)
 __global__ void k_start_loop(Ws w) {
   w.status[ST_PEAK] = w.status[ST_CUR];
-  w.control[C_LOOP] = (w.status[ST_CUR] > 0 && w.status[ST_ERR] == 0) ? 1 : 0;
+  int loop = (w.status[ST_CUR] > 0 && w.status[ST_ERR] == 0) ? 1 : 0;
+  w.control[C_LOOP] = loop;
+  w.control[C_SORTN] = loop ? w.status[ST_CUR] : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -260,7 +303,8 @@ __device__ bool separated(RingRef a, int an, RingRef b, int bn) {
 }
 
 // split_at (kernels.py:179-203): keep value(s, x - anchor) >= 0 in `in` and
-// <= 0 in `out`, both cleaned; counts past MAXV are reported, not stored.
+// <= 0 in `out`, both cleaned and made strictly convex; counts past MAXV are
+// reported, not stored.
 __device__ void split_at(RingRef p, int n, double3 s, double2 anchor, RingRef in, int &in_n, RingRef out,
                          int &out_n) {
   in_n = 0;
@@ -286,8 +330,8 @@ __device__ void split_at(RingRef p, int n, double3 s, double2 anchor, RingRef in
       out_n += 1;
     }
   }
-  in_n = clean_slot(in.base, (int)in.stride, in_n);
-  out_n = clean_slot(out.base, (int)out.stride, out_n);
+  in_n = convexify_ring(in, clean_slot(in.base, (int)in.stride, in_n));
+  out_n = convexify_ring(out, clean_slot(out.base, (int)out.stride, out_n));
 }
 
 // cut (kernels.py:735-812): subtract `cand` (nearer where delta < 0 on the
@@ -391,13 +435,12 @@ __device__ int cut(const Ws &w, int worker, int &vis, int nv, int &nh, RingRef c
   return count;
 }
 
-// peel (kernels.py:815-908): one pass; `cur` is the residual pool of this
-// layer, its other half receives the next residual.
-__global__ __launch_bounds__(256) void k_peel(Ws w, int cur) {
+// peel (kernels.py:815-908): one pass over the ordered pieces of pool 0; the
+// hidden pieces are appended to pool 1 with their sort keys.
+__global__ __launch_bounds__(256) void k_peel(Ws w) {
   if (load_i(w.control + C_LOOP) == 0) return;
   int worker = blockIdx.x * blockDim.x + threadIdx.x;
   const int layer = w.status[ST_PASS];
-  const int nxt = cur ^ 1;
   const long R = w.cap.frontier;
   long long targets = 0, tested_all = 0, hidden_all = 0, ties = 0;
   int max_hidden = 0;
@@ -405,14 +448,14 @@ __global__ __launch_bounds__(256) void k_peel(Ws w, int cur) {
   for (int target = worker; worker < w.workers && target < load_i(w.status + ST_CUR) &&
                             load_i(w.status + ST_ERR) == 0;
        target += w.workers) {
-    RingRef p = residual_ring(w, cur, target);
-    int pn = w.res_n[cur][target];
-    int source = w.res_src[cur][target];
+    RingRef p = residual_ring(w, 0, target);
+    int pn = w.res_n[0][target];
+    int source = w.res_src[0][target];
     int vis = 0, nv = 1, nh = 0;
     copy_ring(p, pn, pool_ring(w, 0, 0, worker));
     pool_n(w, 0, 0, worker) = pn;
     double4 score = w.scores[source];
-    Box tb = ring_box(p, pn, score);
+    Box tb = w.res_box[0][target];
     float4 lo = tb.lo, hi = tb.hi;
     lo.z = -1.0e30f;
     long long tested = 0;
@@ -430,13 +473,13 @@ __global__ __launch_bounds__(256) void k_peel(Ws w, int cur) {
         nv = cut(w, worker, vis, nv, nh, source_ring(w, c), w.src_n[c], delta, source, c, ties);
       }
       // this source's live residual pieces (active_bounds, kernels.py:700-714)
-      for (int run = w.heads[cur][c]; run >= 0 && nv > 0; run = w.res_link[cur][run]) {
-        int end = run + w.res_run[cur][run];
-        for (int k = run; k < end && nv > 0; ++k) {
-          if (!overlaps(w.res_box[cur][k], lo, hi)) continue;
-          tested += 1;
-          nv = cut(w, worker, vis, nv, nh, residual_ring(w, cur, k), w.res_n[cur][k], delta, source, c, ties);
-        }
+      int first = w.run_first[c];
+      if (first < 0) continue;
+      int last = w.run_last[c];
+      for (int k = first; k <= last && nv > 0; ++k) {
+        if (!overlaps(w.res_box[0][k], lo, hi)) continue;
+        tested += 1;
+        nv = cut(w, worker, vis, nv, nh, residual_ring(w, 0, k), w.res_n[0][k], delta, source, c, ties);
       }
     }
     targets += 1;
@@ -454,16 +497,18 @@ __global__ __launch_bounds__(256) void k_peel(Ws w, int cur) {
       int offset = atomicAdd(w.status + ST_NEXT, nh);
       if ((long)offset + nh > R) {
         set_error(w.status, ERR_RESIDUAL);
-      } else if (nh > 0) {
+      } else {
         for (int j = 0; j < nh; ++j) {
           RingRef h = pool_ring(w, 2, j, worker);
           int hn = pool_n(w, 2, j, worker);
-          copy_ring(h, hn, residual_ring(w, nxt, offset + j));
-          w.res_n[nxt][offset + j] = hn;
-          w.res_src[nxt][offset + j] = source;
-          w.res_box[nxt][offset + j] = ring_box(h, hn, score);
+          int slot = offset + j;
+          copy_ring(h, hn, residual_ring(w, 1, slot));
+          w.res_n[1][slot] = hn;
+          w.res_src[1][slot] = source;
+          w.res_box[1][slot] = ring_box(h, hn, score);
+          w.res_keys[slot] = residual_key(w, source, target);
+          w.res_ids[slot] = slot;
         }
-        push_residual(w, nxt, offset, nh, source);
       }
     }
   }
@@ -476,13 +521,6 @@ __global__ __launch_bounds__(256) void k_peel(Ws w, int cur) {
   if (ties) atomicAdd(w.status + ST_TIES, (int)ties);
 }
 
-// Clear the run heads of the pool that receives the next residual.
-__global__ void k_heads_reset(Ws w, int par) {
-  if (load_i(w.control + C_LOOP) == 0) return;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < w.nf; j += gridDim.x * blockDim.x)
-    w.heads[par][j] = -1;
-}
-
 // advance (kernels.py:924-938)
 __global__ void k_layer_advance(Ws w) {
   if (w.control[C_LOOP] == 0) return;
@@ -492,12 +530,66 @@ __global__ void k_layer_advance(Ws w) {
   s[ST_NEXT] = 0;
   s[ST_PASS] += 1;
   if (s[ST_CUR] > 0 && s[ST_PASS] >= w.limit && w.all_layers) s[ST_ERR] |= ERR_LAYERS;
-  w.control[C_LOOP] = (s[ST_CUR] > 0 && s[ST_PASS] < w.limit && s[ST_ERR] == 0) ? 1 : 0;
+  int loop = (s[ST_CUR] > 0 && s[ST_PASS] < w.limit && s[ST_ERR] == 0) ? 1 : 0;
+  w.control[C_LOOP] = loop;
+  w.control[C_SORTN] = loop ? s[ST_CUR] : 0;
+}
+
+// Move the appended pieces of pool 1 to pool 0 in sorted order.
+__global__ void k_residual_gather(Ws w) {
+  if (load_i(w.control + C_LOOP) == 0) return;
+  const int n = w.status[ST_CUR];
+  const long R = w.cap.frontier;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int from = w.res_ids[i];
+    int pn = w.res_n[1][from];
+    const double2 *a = w.res_xy[1] + from;
+    double2 *b = w.res_xy[0] + i;
+    for (int j = 0; j < pn && j < MAXV; ++j) b[j * R] = a[j * R];
+    w.res_n[0][i] = pn;
+    w.res_src[0][i] = w.res_src[1][from];
+    w.res_box[0][i] = w.res_box[1][from];
+  }
+}
+
+// Clear every source's run of pool 0.
+__global__ void k_runs_reset(Ws w) {
+  if (load_i(w.control + C_LOOP) == 0) return;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < w.nf; j += gridDim.x * blockDim.x) {
+    w.run_first[j] = -1;
+    w.run_last[j] = -1;
+  }
+}
+
+// Each source's pieces are consecutive in pool 0: record where they start and end.
+__global__ void k_runs(Ws w) {
+  if (load_i(w.control + C_LOOP) == 0) return;
+  const int n = w.status[ST_CUR];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int s = w.res_src[0][i];
+    if (i == 0 || w.res_src[0][i - 1] != s) w.run_first[s] = i;
+    if (i == n - 1 || w.res_src[0][i + 1] != s) w.run_last[s] = i;
+  }
 }
 
 static inline dim3 grid_for(long n, int block = 256) {
   long g = (n + block - 1) / block;
   return dim3((unsigned)(g < 1 ? 1 : g));
+}
+
+static inline dim3 capped_grid(long n, long cap) {
+  long g = (n + 255) / 256;
+  return dim3((unsigned)(g > cap ? cap : (g < 1 ? 1 : g)));
+}
+
+// Sort the appended pieces by (source, parent slot), move them to pool 0 and
+// record each source's run (launches that return at once after the last pass).
+static void order_residual(const Ws &w, hipStream_t s) {
+  radix_sort(s, w.res_keys, w.res_ids, w.res_keys_alt, w.res_ids_alt, w.control + C_SORTN, 0, w.cap.frontier, 0,
+             w.res_key_bits, w.sort);
+  hipLaunchKernelGGL(k_residual_gather, capped_grid(w.cap.frontier, 8192), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_runs_reset, capped_grid(w.nf, 256), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_runs, capped_grid(w.cap.frontier, 8192), dim3(256), 0, s, w);
 }
 
 // Every pass is in the graph (no conditional nodes): a pass after the last
@@ -514,14 +606,12 @@ void enqueue_layered(const Ws &w, hipStream_t s, Timeline *t) {
     hipLaunchKernelGGL(k_classify, grid_for(w.nf), dim3(256), 0, s, w);
   }
   hipLaunchKernelGGL(k_start_loop, dim3(1), dim3(1), 0, s, w);
+  order_residual(w, s);
   mark(t, s, "k_classify");
-  long hg = (w.nf + 255) / 256;
-  dim3 head_grid((unsigned)(hg > 256 ? 256 : (hg < 1 ? 1 : hg)));
   for (int layer = 0; layer < w.limit; ++layer) {
-    int cur = layer & 1;
-    hipLaunchKernelGGL(k_heads_reset, head_grid, dim3(256), 0, s, w, cur ^ 1);
-    hipLaunchKernelGGL(k_peel, grid_for(w.workers), dim3(256), 0, s, w, cur);
+    hipLaunchKernelGGL(k_peel, grid_for(w.workers), dim3(256), 0, s, w);
     hipLaunchKernelGGL(k_layer_advance, dim3(1), dim3(1), 0, s, w);
+    order_residual(w, s);
   }
   mark(t, s, "k_peel");
 }

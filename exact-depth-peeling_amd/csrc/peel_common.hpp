@@ -103,6 +103,7 @@ enum Control : int {
   C_LEADERS = 141,   // ray-group leaders (they sort first in the ray order)
   C_GRID = 142,      // cells per axis of the boundary-vertex grid
   C_LIVE = 143,      // patches left after convex coalescing
+  C_SORTN = 144,     // layered route: residual pieces to order (0 once the loop ends)
   C_HUGE_SYNC = 160, // arrivals at the grid barriers of the huge-group sort
   C_COUNT = 256,
 };

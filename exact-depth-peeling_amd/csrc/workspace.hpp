@@ -134,12 +134,16 @@ struct Ws {
   int workers, pieces;      // worker threads and pieces per worker pool
   int *ranks;               // [nf] whole-face rank, -1 = residual
   Box *contour_boxes;       // [nf] boxes of each source's contour edges
-  double2 *res_xy[2];       // [MAXV][cap.frontier] ping-pong residual pools
+  // residual pools [MAXV][cap.frontier]: pool 0 holds the pieces of the
+  // current pass in (source, parent slot) order, pool 1 receives appends
+  double2 *res_xy[2];
   int *res_n[2], *res_src[2];
-  int *res_run[2];          // length of the run of pieces starting at a slot
-  int *res_link[2];         // next run of the same source (-1 ends)
   Box *res_box[2];
-  int *heads[2];            // [nf] first run of each source's residual pieces
+  uint64_t *res_keys, *res_keys_alt;  // [cap.frontier] (source, parent slot) of appended pieces
+  int *res_ids, *res_ids_alt;         // [cap.frontier] their append slots
+  int res_parent_bits;      // bits of the parent slot in a key
+  int res_key_bits;         // bits of a whole key
+  int *run_first, *run_last;  // [nf] each source's pieces in pool 0 (-1: none)
   double2 *pool_xy;         // [3][pieces + 3][MAXV][workers] visible, temporary, hidden
   int *pool_n;              // [3][pieces + 3][workers]
   long long *stats;         // [cap.layers][3] targets, candidate tests, fully hidden

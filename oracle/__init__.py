@@ -276,10 +276,20 @@ class LayeredResult:
         self.counts, self.stats = counts, stats
 
 
+def convexify(ring):
+    """The layered route's strict-convexity rule applied to one ring (n, 2)."""
+    xy = np.ascontiguousarray(ring, dtype=np.float64).reshape(-1, 2).copy()
+    n = lib().oracle_convexify(_p(xy), len(xy))
+    return xy[:n]
+
+
 def cpu_layered_peel(vertices, faces, camera, settings, layers=0, layer_capacity=128, pieces=None, residual=None,
-                     cap_poly=None, order_seed=0):
+                     cap_poly=None, order_seed=0, convexify_rings=True):
     """CPU restatement of the layered (DevicePeeler) route in the device's
-    candidate order; ``order_seed`` != 0 shuffles each pass's residual order."""
+    candidate order; ``order_seed`` != 0 shuffles each pass's residual order;
+    ``convexify_rings=False`` drops the strict-convexity rule (to show what it
+    prevents)."""
+    lib().oracle_cpu_layered_options(int(bool(convexify_rings)))
     vertices = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
     faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
     camera = np.ascontiguousarray(camera, dtype=np.float64).reshape(4, 4)
@@ -303,3 +313,61 @@ def cpu_layered_peel(vertices, faces, camera, settings, layers=0, layer_capacity
     P = min(c["polygons"], cap_poly)
     rings = [poly_xy[i, : poly_n[i]].copy() for i in range(P)]
     return LayeredResult(rings, poly_src[:P].copy(), poly_layer[:P].copy(), c, stats[: c["passes"]].copy())
+
+
+FAST_COUNT_FIELDS = ("layers", "patches", "faces", "error", "constant_rank_sources", "residual_sources",
+                     "candidate_tests", "polygon_differences", "convexity_repairs", "collapsed_polygons",
+                     "depth_ties", "active_sources", "contour_segments", "boundary_edges", "silhouette_edges",
+                     "nonmanifold_or_inconsistent_edges", "same_facing_internal_edges_ignored",
+                     "camera_clipping_boundary_edges", "point_query_candidates")
+
+
+class FastResult:
+    def __init__(self, rings, sources, layers, tri_uv, tri_src, tri_layer, counts, setup_s, extract_s):
+        self.rings, self.sources, self.layers = rings, sources, layers
+        self.tri_uv, self.tri_src, self.tri_layer = tri_uv, tri_src, tri_layer
+        self.counts, self.setup_s, self.extract_s = counts, setup_s, extract_s
+
+
+def fast_peel(vertices, faces, camera, settings, layers=0, threads=1, repeat=1, bits=30, keep=True,
+              cap_poly=None, cap_tri=None, max_vertices=64):
+    """The restatement of the reference's fast CPU method (fast-bvh-peel
+    --resolved-mesh). ``extract_s`` lists the seconds of each of ``repeat``
+    extractions (the reference's algorithm scope); ``setup_s`` the projection
+    and topology before them."""
+    vertices = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+    faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+    camera = np.ascontiguousarray(camera, dtype=np.float64).reshape(4, 4)
+    settings = np.ascontiguousarray(settings, dtype=np.float64).reshape(6)
+    n = max(16, len(faces))
+    cap_poly = cap_poly or 64 * n
+    cap_tri = cap_tri or 256 * n
+    counts = np.zeros(len(FAST_COUNT_FIELDS), np.int64)
+    seconds = np.zeros(1 + repeat)
+    if keep:
+        poly_xy = np.zeros((cap_poly, max_vertices, 2))
+        poly_n = np.zeros(cap_poly, np.int32)
+        poly_src = np.zeros(cap_poly, np.int32)
+        poly_layer = np.zeros(cap_poly, np.int32)
+        tri_uv = np.zeros((cap_tri, 3, 2))
+        tri_src = np.zeros(cap_tri, np.int32)
+        tri_layer = np.zeros(cap_tri, np.int32)
+        ptrs = [_p(poly_xy), _p(poly_n), _p(poly_src), _p(poly_layer), int(cap_poly), int(max_vertices),
+                _p(tri_uv), _p(tri_src), _p(tri_layer), int(cap_tri)]
+    else:
+        ptrs = [None, None, None, None, 0, int(max_vertices), None, None, None, 0]
+    f = lib().oracle_fast_peel
+    f.restype = ctypes.c_int
+    rc = f(_p(vertices), len(vertices), _p(faces), len(faces), _p(camera), _p(settings), int(layers), int(threads),
+           int(repeat), int(bits), *ptrs, _p(counts), _p(seconds))
+    if rc:
+        raise ValueError("fast_peel: bits must be in [12, 30] and repeat positive")
+    c = dict(zip(FAST_COUNT_FIELDS, map(int, counts)))
+    if not keep:
+        return FastResult(None, None, None, None, None, None, c, float(seconds[0]), seconds[1:].tolist())
+    P, T = min(c["patches"], cap_poly), min(c["faces"], cap_tri)
+    if (poly_n[:P] > max_vertices).any():
+        raise ValueError("a fast-route patch exceeds max_vertices")
+    rings = [poly_xy[i, : poly_n[i]].copy() for i in range(P)]
+    return FastResult(rings, poly_src[:P].copy(), poly_layer[:P].copy(), tri_uv[:T].copy(), tri_src[:T].copy(),
+                      tri_layer[:T].copy(), c, float(seconds[0]), seconds[1:].tolist())

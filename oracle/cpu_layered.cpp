@@ -80,8 +80,9 @@ bool separated(const Ring &a, const Ring &b) {
 // a rotated line and misplaces a strip up to ~1e-9 wide.) Not in the
 // reference; the device's layered route applies the same rule.
 constexpr double TURN_TOL = 1.0e-12;
+bool g_convexify = true;  // off only to show what the rule prevents
 void convexify(Ring &p) {
-  if (p.n > MAXV) return;
+  if (p.n > MAXV || !g_convexify) return;
   bool changed = true;
   while (changed && p.n >= 3) {
     changed = false;
@@ -201,6 +202,22 @@ struct Piece {
 }  // namespace
 
 extern "C" {
+
+void oracle_cpu_layered_options(int convexify_rings) { g_convexify = convexify_rings != 0; }
+
+// Convexify one ring of n <= 32 vertices (xy interleaved) in place; returns
+// the new count.
+int oracle_convexify(double *xy, int n) {
+  Ring r;
+  for (int j = 0; j < n && j < MAXV; ++j) r.v[j] = V2{xy[2 * j], xy[2 * j + 1]};
+  r.n = n;
+  bool saved = g_convexify;
+  g_convexify = true;
+  convexify(r);
+  g_convexify = saved;
+  for (int j = 0; j < r.n; ++j) xy[2 * j] = r.v[j].x, xy[2 * j + 1] = r.v[j].y;
+  return r.n;
+}
 
 // counts[0..9]: polygons, layers, error bits, depth ties, passes, peak
 // residual pieces, max hidden pieces of one target, whole sources emitted by

@@ -181,3 +181,63 @@ def test_device_route_matches_event_route(grid):
     # the per-layer cuts leave near-collinear vertex runs on this mesh; one
     # fan triangle of ~1e-15 x 1e-3 has a noise-level negative area (-3.8e-20)
     check_extraction(d, host[:4], grid=32, triangles=grid[0] <= 32, orientation_tol=1e-18)
+
+
+def layered_records(rings, srcs, lays):
+    """Multiset of (source, layer, ring bytes) of the non-empty fragments."""
+    return sorted((int(s), int(l), np.ascontiguousarray(r).tobytes()) for r, s, l in zip(rings, srcs, lays) if len(r))
+
+
+def _device_vs_cpu(host, cap=None):
+    from hip_peel import DevicePeeler
+    from hip_peel.io import upload
+
+    p = DevicePeeler(*upload(*host[:4]), capacities=cap)
+    p.run()
+    info = p.read_status()
+    c = p.cap
+    cpu = oracle.cpu_layered_peel(*host[:4], layer_capacity=c.layers, pieces=c.pieces_per_worker,
+                                  residual=c.residual, cap_poly=c.polygons)
+    assert cpu.counts["error"] == 0
+    assert (info["layers"], info["peeling_passes"], info["depth_ties"]) == (
+        cpu.counts["layers"], cpu.counts["passes"], cpu.counts["depth_ties"])
+    assert info["peak_residual_patches"] == cpu.counts["peak_residual"]
+    stats = p.stats.cpu().numpy()[: info["peeling_passes"]]
+    np.testing.assert_array_equal(stats, cpu.stats)
+    rings, srcs, lays = fragments(p, info)
+    assert layered_records(rings, srcs, lays) == layered_records(cpu.rings, cpu.sources, cpu.layers)
+    return p, info
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_device_matches_cpu_restatement(name):
+    """Bit parity with the CPU restatement of the layered route: every piece
+    is cut in the same candidate order with the same binary64 operations, so
+    the fragment multisets are identical, as are the per-layer statistics."""
+    print(f"[device-cpu] {name}", flush=True)
+    _device_vs_cpu(oracle.load_scene(fixture_path(name))[:4])
+
+
+def test_device_matches_cpu_restatement_torus_knot():
+    from hip_peel import scenes
+
+    _device_vs_cpu(scenes.torus_knot(segments=32, sides=16))
+
+
+def test_device_resolved_soup_exact_ranks():
+    """The resolved 48-triangle random soup (3,024 pieces, 16 layers): bit
+    parity with the CPU restatement, and every fragment on its exact rank with
+    no allowance (the strict-convexity rule of the layered route's clipping;
+    without it some candidate orders left ~3e-10-wide strips one layer off)."""
+    from hip_peel import Capacities, scenes
+
+    host = scenes.random_triangles_resolved(48, 0, workers=8)[:4]
+    F = len(host[1])
+    cap = replace(Capacities.for_mesh(F), residual=64 * F, polygons=64 * F, vertices=512 * F, triangles=512 * F,
+                  pieces_per_worker=4096)
+    p, info = _device_vs_cpu(host, cap)
+    assert info["layers"] == 16
+    # fan triangles over near-collinear vertex runs carry noise-level doubled
+    # areas (~-2e-20), as on the torus knot above
+    report = check_extraction(p, host, grid=48, orientation_tol=1e-18)
+    print(f"[device-cpu] soup 48: {report}", flush=True)

@@ -76,31 +76,6 @@ def rank_mismatches(host, rings, sources, layers, parents=None):
     return bad, checked
 
 
-# The layered route (warp_peel/kernels.py:735-812) cuts each target by its
-# occluders in an order set by atomic pushes of the residual lists, so its
-# piece decompositions differ from run to run. On resolved random soups some
-# runs emit a few slivers one layer off along the long shared edges of
-# coplanar pieces of one parent triangle (observed over 8 runs of the 48-
-# triangle soup: 2 untiled runs with areas up to 5.5e-12, 6 tiled runs with
-# areas up to 3.7e-10 and widths up to 5e-8 of the window; every other
-# fragment exact; the cause is not identified). Such fragments are admitted,
-# and reported, below the route's own area tolerance; the event route gets no
-# allowance.
-LAYERED_SLIVER = 1e-9
-
-
-def sliver_filter(bad, areas, route):
-    """Split rank mismatches into admitted slivers and real errors."""
-    if route != "layered":
-        return bad, []
-    frags = sorted(set(b[0] for b in bad))
-    slivers = [f for f in frags if abs(areas[f]) < LAYERED_SLIVER]
-    if slivers:
-        print(f"[tiles] layered slivers with a wrong rank: {len(slivers)}, areas "
-              f"{sorted(float(abs(areas[f])) for f in slivers)[-3:]}", flush=True)
-    return [b for b in bad if b[0] not in set(slivers)], slivers
-
-
 def tile_audit(host, tile, points):
     """Grid audit of one tile at the points strictly inside it; returns how
     many were audited. A tile without fragments must have no covering source
@@ -185,7 +160,9 @@ def test_tiled_matches_untiled(make, route):
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 if route == "events" else 1e-9)
     assert_counter_clockwise(rv, off)
     bad, checked = rank_mismatches(host, np.split(rv, off[1:-1]), src, lay)
-    bad, _ = sliver_filter(bad, ring_areas(rv, off), route)
+    assert checked > 0 and not bad, bad[:5]
+    # the untiled run carries the exact ranks too
+    bad, checked = rank_mismatches(host, np.split(ref_vert, ref_off[1:-1]), ref_src, ref_lay)
     assert checked > 0 and not bad, bad[:5]
     points = grid_points(48)
     audited = sum(tile_audit(host, t, points) for t in tiled.results)
