@@ -12,6 +12,8 @@ settings (6,) f64) in the peelers' input layout:
   ~55K faces each (717,740 total), orthographic along z.
 """
 
+import functools
+
 import numpy as np
 
 
@@ -177,36 +179,72 @@ def icosahedron():
     return v / np.linalg.norm(v, axis=1)[:, None], f
 
 
+def _lattice_triangles(m):
+    """Lattice (a, b) of the three corners of every small triangle of one
+    icosahedron face split m x m, in row order: for each a, the upright
+    triangle at b followed by the inverted one at b (none after the last b)."""
+    rows = []
+    for a in range(m):
+        b = np.arange(m - a)
+        up = np.stack([np.stack([np.full_like(b, a), b], 1), np.stack([np.full_like(b, a + 1), b], 1),
+                       np.stack([np.full_like(b, a), b + 1], 1)], 1)
+        c = b[:-1]
+        down = np.stack([np.stack([np.full_like(c, a + 1), c], 1), np.stack([np.full_like(c, a + 1), c + 1], 1),
+                         np.stack([np.full_like(c, a), c + 1], 1)], 1)
+        both = np.empty((len(b) + len(c), 3, 2), dtype=np.int64)
+        both[0::2], both[1::2] = up, down
+        rows.append(both)
+    return np.concatenate(rows)
+
+
+@functools.lru_cache(maxsize=16)
+def _geodesic_lattice(m):
+    """Unit directions and faces of the class-I geodesic sphere of frequency m
+    (before orientation): a lattice point is identified by its integer
+    barycentric weights on the icosahedron corners, so points shared by
+    neighbouring faces are one vertex (closed mesh), numbered in order of first
+    use; its position is the weighted sum of its corners in ascending corner
+    order, divided by m, then normalised."""
+    if not 1 <= m < 128:
+        raise ValueError("geodesic_sphere: 1 <= m < 128 (weights are 7-bit fields of the point keys)")
+    corners, ico = icosahedron()
+    tri = _lattice_triangles(m)
+    a, b = tri[..., 0].ravel(), tri[..., 1].ravel()
+    n = len(a)
+    ids = np.repeat(ico.astype(np.int64), n, axis=0).reshape(len(ico), n, 3)
+    w = np.stack([np.broadcast_to(m - a - b, (len(ico), n)), np.broadcast_to(a, (len(ico), n)),
+                  np.broadcast_to(b, (len(ico), n))], axis=-1)
+    # key: the (corner << 7 | weight) pairs with positive weight, ascending;
+    # absent pairs are 15 << 7 so that they sort last
+    pair = np.where(w > 0, (ids << 7) | w, 15 << 7)
+    pair.sort(axis=-1)
+    key = (pair[..., 0] << 22) | (pair[..., 1] << 11) | pair[..., 2]
+    unique, first, inverse = np.unique(key.ravel(), return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    number = np.empty(len(order), dtype=np.int64)
+    number[order] = np.arange(len(order))
+    faces = number[inverse].reshape(-1, 3).astype(np.int32)
+    k = unique[order]
+    p = np.zeros((len(k), 3))
+    for shift in (22, 11, 0):
+        q = (k >> shift) & 0x7FF
+        corner, weight = q >> 7, (q & 0x7F).astype(np.float64)
+        p = np.where((corner < 12)[:, None], p + weight[:, None] * corners[np.minimum(corner, 11)], p)
+    p = p / m
+    dirs = p / np.array([np.linalg.norm(x) for x in p])[:, None]
+    dirs.flags.writeable = False
+    faces.flags.writeable = False
+    return dirs, faces
+
+
 def geodesic_sphere(radius, m, amplitude=0.0, seed=0):
     """Class-I geodesic sphere: every icosahedron face split into m^2 triangles
-    (20 m^2 faces), lattice points projected to the sphere. Shared lattice
-    points are keyed by their exact integer barycentric weights, so the mesh
-    is closed and every shared vertex is computed once. Radial value-noise
-    offset of the given amplitude."""
-    corners, ico = icosahedron()
-    index, points, faces = {}, [], []
-
-    def vertex(weights):
-        key = tuple(sorted((c, w) for c, w in weights if w > 0))
-        if key not in index:
-            p = sum(w * corners[c] for c, w in key) / m
-            index[key] = len(points)
-            points.append(p / np.linalg.norm(p))
-        return index[key]
-
-    for A, B, C in ico:
-        def P(a, b):
-            return vertex(((A, m - a - b), (B, a), (C, b)))
-
-        for a in range(m):
-            for b in range(m - a):
-                faces.append((P(a, b), P(a + 1, b), P(a, b + 1)))
-                if a + b < m - 1:
-                    faces.append((P(a + 1, b), P(a + 1, b + 1), P(a, b + 1)))
-    dirs = np.array(points)
+    (20 m^2 faces), lattice points projected to the sphere, every face
+    oriented outward. Radial value-noise offset of the given amplitude."""
+    dirs, faces = _geodesic_lattice(m)
     r = radius + amplitude * _value_noise(dirs * radius, seed)
     vertices = dirs * r[:, None]
-    faces = np.array(faces, dtype=np.int32)
+    faces = faces.copy()
     a, b, c = vertices[faces[:, 0]], vertices[faces[:, 1]], vertices[faces[:, 2]]
     inward = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) < 0
     faces[inward] = faces[inward][:, [0, 2, 1]]

@@ -22,6 +22,10 @@ fields the tests use:
                           reference's validation suite (results/fast/validation.json,
                           field audit.oracle_layers).
 * ``resolved-crossing.json``  the resolver golden (results/fast/resolved-crossing.json).
+* ``fast_records.json``  per fixture, the reference's fast CPU method run on the
+                          resolved-mesh contract (results/fast/validation.json,
+                          records "<case>-mesh"): its mesh topology counts,
+                          its rank classification counts and its layer count.
 """
 
 import json
@@ -78,6 +82,18 @@ def main():
         if audit.get("oracle_layers") is not None:
             layers[record["case"]] = audit["oracle_layers"]
     (OUT / "layers.json").write_text(json.dumps(layers, indent=1, sort_keys=True) + "\n")
+    fast = {}
+    for record in validation["records"]:
+        if not record["case"].endswith("-mesh"):
+            continue
+        cls = record.get("classification") or {}
+        fast[record["case"][: -len("-mesh")]] = {
+            "layers": record["layers"],
+            "topology": {k: v for k, v in record["topology"].items() if k != "resolved_mesh_contract"},
+            "classification": {k: cls[k] for k in ("constant_rank_sources", "sources_requiring_clipping",
+                                                   "point_query_candidates") if k in cls},
+        }
+    (OUT / "fast_records.json").write_text(json.dumps(fast, indent=1, sort_keys=True) + "\n")
     shutil.copyfile(REF / "results/fast/resolved-crossing.json", OUT / "resolved-crossing.json")
 
 

@@ -60,3 +60,49 @@ def test_procedural_scenes_are_closed_meshes():
     v, f, c, s = scenes.stack(64)
     assert len(f) == 64 and np.all(v[3::3, 2] > v[0::3, 2][:-1] - 1)
     assert np.isclose(np.linalg.det(c[:3, :3]), 1.0)
+
+
+def _direct_geodesic(radius, m, amplitude, seed):
+    """Point-by-point construction of the class-I geodesic sphere: lattice
+    points keyed by their (corner, weight) pairs in a dict, numbered in order
+    of first use."""
+    from hip_peel.scenes import _value_noise, icosahedron
+
+    corners, ico = icosahedron()
+    index, points, faces = {}, [], []
+
+    def vertex(*weights):
+        key = tuple(sorted((c, w) for c, w in weights if w > 0))
+        if key not in index:
+            p = sum(w * corners[c] for c, w in key) / m
+            index[key] = len(points)
+            points.append(p / np.linalg.norm(p))
+        return index[key]
+
+    for A, B, C in ico:
+        for a in range(m):
+            for b in range(m - a):
+                faces.append((vertex((A, m - a - b), (B, a), (C, b)), vertex((A, m - a - 1 - b), (B, a + 1), (C, b)),
+                              vertex((A, m - a - b - 1), (B, a), (C, b + 1))))
+                if a + b < m - 1:
+                    faces.append((vertex((A, m - a - 1 - b), (B, a + 1), (C, b)),
+                                  vertex((A, m - a - b - 2), (B, a + 1), (C, b + 1)),
+                                  vertex((A, m - a - b - 1), (B, a), (C, b + 1))))
+    dirs = np.array(points)
+    vertices = dirs * (radius + amplitude * _value_noise(dirs * radius, seed))[:, None]
+    faces = np.array(faces, dtype=np.int32)
+    a, b, c = vertices[faces[:, 0]], vertices[faces[:, 1]], vertices[faces[:, 2]]
+    inward = np.einsum("ij,ij->i", np.cross(b - a, c - a), a + b + c) < 0
+    faces[inward] = faces[inward][:, [0, 2, 1]]
+    return vertices, faces
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 5, 8, 13, 53])
+def test_geodesic_sphere_matches_direct_construction(m):
+    from hip_peel import scenes
+
+    v, f = scenes.geodesic_sphere(1.7, m, amplitude=0.2, seed=m)
+    dv, df = _direct_geodesic(1.7, m, 0.2, m)
+    np.testing.assert_array_equal(v, dv)
+    np.testing.assert_array_equal(f, df)
+    assert len(f) == 20 * m * m and len(v) == 10 * m * m + 2
