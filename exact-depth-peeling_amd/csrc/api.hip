@@ -413,6 +413,27 @@ int peel_capture(PeelWorkspace *ws) {
   return 0;
 }
 
+// Node counts per hipGraphNodeType of a captured graph.
+static int census(hipGraph_t g, int32_t *counts, int32_t n) {
+  if (!g) return fail(9, "no captured graph: capture first");
+  for (int i = 0; i < n; ++i) counts[i] = 0;
+  size_t total = 0;
+  HIP_TRY(hipGraphGetNodes(g, nullptr, &total));
+  std::vector<hipGraphNode_t> nodes(total);
+  if (total) HIP_TRY(hipGraphGetNodes(g, nodes.data(), &total));
+  for (size_t i = 0; i < total; ++i) {
+    hipGraphNodeType type;
+    HIP_TRY(hipGraphNodeGetType(nodes[i], &type));
+    if ((int)type >= 0 && (int)type < n) counts[(int)type] += 1;
+  }
+  return 0;
+}
+
+int peel_graph_census(PeelWorkspace *ws, int32_t *counts, int32_t n) {
+  if (!ws || !counts || n < 1) return fail(1, "null argument");
+  return census(ws->graph, counts, n);
+}
+
 int peel_get_tuning(PeelWorkspace *ws, PeelTuning *t) {
   if (!ws || !t) return fail(1, "null argument");
   const Ws &w = ws->w;
@@ -551,7 +572,11 @@ int peel_batch_create(PeelWorkspace *const *list, int32_t count, PeelBatch **out
   err = hipEventRecord(b->events[0], root);
   for (int i = 1; i < count && err == hipSuccess; ++i) err = hipStreamWaitEvent(list[i]->stream, b->events[0], 0);
   for (int i = 0; i < count && err == hipSuccess; ++i) {
-    enqueue_all(list[i]->w, list[i]->stream, nullptr);
+    // members may run concurrently: no launch of theirs may rely on its
+    // blocks being co-resident
+    Ws member = list[i]->w;
+    member.split_huge = 1;
+    enqueue_all(member, list[i]->stream, nullptr);
     err = hipGetLastError();
   }
   // join: the root waits on every member stream
@@ -581,6 +606,11 @@ int peel_batch_launch(PeelBatch *b) {
   if (!b) return fail(1, "null batch");
   HIP_TRY(hipGraphLaunch(b->exec, b->members[0]->stream));
   return 0;
+}
+
+int peel_batch_graph_census(PeelBatch *b, int32_t *counts, int32_t n) {
+  if (!b || !counts || n < 1) return fail(1, "null argument");
+  return census(b->graph, counts, n);
 }
 
 int peel_batch_synchronize(PeelBatch *b) {

@@ -572,6 +572,28 @@ __global__ void k_runs(Ws w) {
   }
 }
 
+// One ring operation on one lane, for unit tests of the device functions:
+// op 0 clean (both routes), op 1 split_edge by the edge param[0..3] = (a, b)
+// (the layered route's split_at in the edge frame, strictly convex outputs),
+// op 2 convexify_ring. Rings are contiguous [MAXV] double2; out holds two.
+__global__ void k_ring_test(int op, const double2 *xy, int n, const double *param, double2 *out, int *out_n) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  RingRef a{out, 1}, b{out + MAXV, 1};
+  if (op == 1) {
+    RingRef src{const_cast<double2 *>(xy), 1};
+    double2 e0 = make_double2(param[0], param[1]), e1 = make_double2(param[2], param[3]);
+    double2 d = make_double2(e1.x - e0.x, e1.y - e0.y);
+    int in_n, o_n;
+    split_at(src, n, make_double3(-d.y, d.x, 0.0), e0, a, in_n, b, o_n);
+    out_n[0] = in_n;
+    out_n[1] = o_n;
+    return;
+  }
+  for (int j = 0; j < n && j < MAXV; ++j) a.at(j) = xy[j];
+  out_n[0] = op == 0 ? clean_slot(out, 1, n) : convexify_ring(a, n);
+  out_n[1] = 0;
+}
+
 static inline dim3 grid_for(long n, int block = 256) {
   long g = (n + block - 1) / block;
   return dim3((unsigned)(g < 1 ? 1 : g));
@@ -617,3 +639,11 @@ void enqueue_layered(const Ws &w, hipStream_t s, Timeline *t) {
 }
 
 }  // namespace peel
+
+extern "C" int peel_ring_test(int32_t op, const double *xy, int32_t n, const double *param, double *out_xy,
+                              int32_t *out_n, void *stream) {
+  if (op < 0 || op > 2 || n < 0 || n > peel::MAXV || !xy || !out_xy || !out_n || (op == 1 && !param)) return 1;
+  hipLaunchKernelGGL(peel::k_ring_test, dim3(1), dim3(64), 0, static_cast<hipStream_t>(stream), op,
+                     reinterpret_cast<const double2 *>(xy), n, param, reinterpret_cast<double2 *>(out_xy), out_n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

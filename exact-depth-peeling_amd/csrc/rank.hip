@@ -763,20 +763,19 @@ __device__ void huge_barrier(const Ws &w, int &generation) {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(HUGE_THREADS) void k_sort_huge(Ws w) {
-  __shared__ unsigned long long kk[MEDIUM_SEGMENT];
-  __shared__ int ss[MEDIUM_SEGMENT], hh[MEDIUM_SEGMENT];
-  if (w.status[ST_ERR] != 0) return;
-  const int count = load_i(w.control + C_HUGE);
-  if (count == 0) return;
-  int generation = 0;
-  // 1) every chunk of every huge group, one LDS sort each, into (hkeys, hids)
-  long tasks = 0, largest = 0;
+// Phases of the huge-group sort: the LDS chunk sorts, merge pass p of run
+// width MEDIUM_SEGMENT << p, and the copy-back of groups with an odd number
+// of passes. huge_plan reads the groups' sizes.
+__device__ __forceinline__ void huge_plan(const Ws &w, int count, long &tasks, long &largest) {
+  tasks = 0, largest = 0;
   for (int item = 0; item < count; ++item) {
     int c = w.group_counts[w.medium[w.cap.regions - 1 - item]];
     tasks += (c + MEDIUM_SEGMENT - 1) / MEDIUM_SEGMENT;
     largest = max(largest, (long)c);
   }
+}
+
+__device__ void huge_chunks(const Ws &w, int count, long tasks, unsigned long long *kk, int *ss, int *hh) {
   for (long task = blockIdx.x; task < tasks; task += gridDim.x) {
     int item = 0;
     long first = 0;
@@ -835,59 +834,60 @@ __global__ __launch_bounds__(HUGE_THREADS) void k_sort_huge(Ws w) {
     }
     __syncthreads();
   }
-  huge_barrier(w, generation);
-  // 2) merge passes over the whole grid
+}
+
+__device__ void huge_merge(const Ws &w, int count, int pass, long width) {
   const long threads = (long)gridDim.x * blockDim.x;
   const long me = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int pass = 0;
-  for (long width = MEDIUM_SEGMENT; width < largest; width *= 2, ++pass) {
-    for (int item = 0; item < count; ++item) {
-      const int g = w.medium[w.cap.regions - 1 - item];
-      const long c = w.group_counts[g];
-      if (c <= width) continue;
-      const int off = w.group_offsets[g];
-      const uint64_t *kin = (pass & 1 ? w.hkeys_alt : w.hkeys) + off;
-      const int *iin = (pass & 1 ? w.hids_alt : w.hids) + off;
-      uint64_t *kout = (pass & 1 ? w.hkeys : w.hkeys_alt) + off;
-      int *iout = (pass & 1 ? w.hids : w.hids_alt) + off;
-      const long per = (c + threads - 1) / threads;
-      long k = me * per;
-      const long k_end = min(c, k + per);
-      while (k < k_end) {
-        const long a = k / (2 * width) * (2 * width);
-        const long mid_run = min(c, a + width), e = min(c, a + 2 * width);
-        const long nl = mid_run - a, nr = e - mid_run;
-        const uint64_t *lk = kin + a, *rk = kin + mid_run;
-        const int *li = iin + a, *ri = iin + mid_run;
-        // merge-path split of the first d outputs of this pair of runs
-        const long d = k - a;
-        long lo = max(0L, d - nr), hi = min(d, nl);
-        while (lo < hi) {
-          long m = (lo + hi) / 2;
-          if (hit_before(w, lk[m], li[m], rk[d - m - 1], ri[d - m - 1]))
-            lo = m + 1;
-          else
-            hi = m;
-        }
-        long i = lo, j = d - lo;
-        const long stop = min(k_end, e);
-        for (; k < stop; ++k) {
-          bool left = j >= nr || (i < nl && hit_before(w, lk[i], li[i], rk[j], ri[j]));
-          if (left) {
-            kout[k] = lk[i];
-            iout[k] = li[i];
-            ++i;
-          } else {
-            kout[k] = rk[j];
-            iout[k] = ri[j];
-            ++j;
-          }
+  for (int item = 0; item < count; ++item) {
+    const int g = w.medium[w.cap.regions - 1 - item];
+    const long c = w.group_counts[g];
+    if (c <= width) continue;
+    const int off = w.group_offsets[g];
+    const uint64_t *kin = (pass & 1 ? w.hkeys_alt : w.hkeys) + off;
+    const int *iin = (pass & 1 ? w.hids_alt : w.hids) + off;
+    uint64_t *kout = (pass & 1 ? w.hkeys : w.hkeys_alt) + off;
+    int *iout = (pass & 1 ? w.hids : w.hids_alt) + off;
+    const long per = (c + threads - 1) / threads;
+    long k = me * per;
+    const long k_end = min(c, k + per);
+    while (k < k_end) {
+      const long a = k / (2 * width) * (2 * width);
+      const long mid_run = min(c, a + width), e = min(c, a + 2 * width);
+      const long nl = mid_run - a, nr = e - mid_run;
+      const uint64_t *lk = kin + a, *rk = kin + mid_run;
+      const int *li = iin + a, *ri = iin + mid_run;
+      // merge-path split of the first d outputs of this pair of runs
+      const long d = k - a;
+      long lo = max(0L, d - nr), hi = min(d, nl);
+      while (lo < hi) {
+        long m = (lo + hi) / 2;
+        if (hit_before(w, lk[m], li[m], rk[d - m - 1], ri[d - m - 1]))
+          lo = m + 1;
+        else
+          hi = m;
+      }
+      long i = lo, j = d - lo;
+      const long stop = min(k_end, e);
+      for (; k < stop; ++k) {
+        bool left = j >= nr || (i < nl && hit_before(w, lk[i], li[i], rk[j], ri[j]));
+        if (left) {
+          kout[k] = lk[i];
+          iout[k] = li[i];
+          ++i;
+        } else {
+          kout[k] = rk[j];
+          iout[k] = ri[j];
+          ++j;
         }
       }
     }
-    huge_barrier(w, generation);
   }
-  // 3) a group with an odd number of passes holds its order in hids_alt
+}
+
+__device__ void huge_finish(const Ws &w, int count) {
+  const long threads = (long)gridDim.x * blockDim.x;
+  const long me = (long)blockIdx.x * blockDim.x + threadIdx.x;
   for (int item = 0; item < count; ++item) {
     const int g = w.medium[w.cap.regions - 1 - item];
     const long c = w.group_counts[g];
@@ -897,6 +897,52 @@ __global__ __launch_bounds__(HUGE_THREADS) void k_sort_huge(Ws w) {
       const int off = w.group_offsets[g];
       for (long i = me; i < c; i += threads) w.hids[off + i] = w.hids_alt[off + i];
     }
+  }
+}
+
+// One launch for every phase, with a grid barrier between phases: its
+// HUGE_BLOCKS blocks are all resident when it runs alone after the previous
+// kernel of its stream (the single-workspace graph).
+__global__ __launch_bounds__(HUGE_THREADS) void k_sort_huge(Ws w) {
+  __shared__ unsigned long long kk[MEDIUM_SEGMENT];
+  __shared__ int ss[MEDIUM_SEGMENT], hh[MEDIUM_SEGMENT];
+  if (w.status[ST_ERR] != 0) return;
+  const int count = load_i(w.control + C_HUGE);
+  if (count == 0) return;
+  int generation = 0;
+  long tasks, largest;
+  huge_plan(w, count, tasks, largest);
+  huge_chunks(w, count, tasks, kk, ss, hh);
+  huge_barrier(w, generation);
+  int pass = 0;
+  for (long width = MEDIUM_SEGMENT; width < largest; width *= 2, ++pass) {
+    huge_merge(w, count, pass, width);
+    huge_barrier(w, generation);
+  }
+  huge_finish(w, count);
+}
+
+// One phase per launch, no grid barrier (graphs whose members run
+// concurrently, where the blocks of one launch need not be co-resident):
+// phase 0 the chunk sorts, phase 1 + p merge pass p (nothing to do once the
+// runs cover the largest group), phase HUGE_FINISH the copy-back.
+constexpr int HUGE_FINISH = 1 << 20;
+__global__ __launch_bounds__(HUGE_THREADS) void k_sort_huge_phase(Ws w, int phase) {
+  __shared__ unsigned long long kk[MEDIUM_SEGMENT];
+  __shared__ int ss[MEDIUM_SEGMENT], hh[MEDIUM_SEGMENT];
+  if (w.status[ST_ERR] != 0) return;
+  const int count = load_i(w.control + C_HUGE);
+  if (count == 0) return;
+  long tasks, largest;
+  huge_plan(w, count, tasks, largest);
+  if (phase == 0) {
+    huge_chunks(w, count, tasks, kk, ss, hh);
+  } else if (phase == HUGE_FINISH) {
+    huge_finish(w, count);
+  } else {
+    const int pass = phase - 1;
+    const long width = (long)MEDIUM_SEGMENT << pass;
+    if (width < largest) huge_merge(w, count, pass, width);
   }
 }
 
@@ -978,7 +1024,16 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   hipLaunchKernelGGL(k_hit_scatter, stride_grid(w.cap.hits), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_sort_small, stride_grid(w.cap.regions, 4), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_sort_medium, dim3(1024), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_sort_huge, dim3(HUGE_BLOCKS), dim3(HUGE_THREADS), 0, s, w);
+  if (!w.split_huge) {
+    hipLaunchKernelGGL(k_sort_huge, dim3(HUGE_BLOCKS), dim3(HUGE_THREADS), 0, s, w);
+  } else {
+    // enough merge passes for a group as large as the hit pool
+    int passes = 0;
+    for (long width = MEDIUM_SEGMENT; width < w.cap.hits; width *= 2) ++passes;
+    for (int phase = 0; phase <= passes; ++phase)
+      hipLaunchKernelGGL(k_sort_huge_phase, dim3(HUGE_BLOCKS), dim3(HUGE_THREADS), 0, s, w, phase);
+    hipLaunchKernelGGL(k_sort_huge_phase, dim3(HUGE_BLOCKS), dim3(HUGE_THREADS), 0, s, w, HUGE_FINISH);
+  }
   mark(t, s, "hit_sort");
   hipLaunchKernelGGL(k_assign, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   mark(t, s, "k_assign");

@@ -157,6 +157,9 @@ struct Ws {
   int ray_lanes_target;
   int heavy_lanes;
   int long_edge_cells;
+  // hit groups above 2,048 are sorted one phase per launch instead of by one
+  // launch with grid barriers (set for graphs whose members run concurrently)
+  int split_huge;
 };
 
 // Optional host-side timeline: hipEvents recorded between stages when the

@@ -118,6 +118,9 @@ EXPORTS = (
     "pool_mlp_gradient",
     "pool_mlp_adam",
     "peel_extract_static",
+    "peel_graph_census",
+    "peel_batch_graph_census",
+    "peel_ring_test",
 )
 
 _lib = None
@@ -167,10 +170,27 @@ def load():
     lib.pool_mlp_adam.restype = ctypes.c_int
     lib.peel_extract_static.argtypes = [ctypes.POINTER(PeelDesc), ctypes.POINTER(PeelOutputs), vp]
     lib.peel_extract_static.restype = ctypes.c_int
+    lib.peel_ring_test.argtypes = [i32, vp, i32, vp, vp, vp, vp]
+    lib.peel_ring_test.restype = ctypes.c_int
+    for name in ("peel_graph_census", "peel_batch_graph_census"):
+        getattr(lib, name).argtypes = [vp, vp, i32]
+        getattr(lib, name).restype = ctypes.c_int
     lib.peel_last_error.restype = ctypes.c_char_p
     lib.peel_version.restype = ctypes.c_char_p
     _lib = lib
     return lib
+
+
+GRAPH_NODE_TYPES = ("kernel", "memcpy", "memset", "host", "graph", "empty", "wait_event", "event_record",
+                    "semaphore_signal", "semaphore_wait", "mem_alloc", "mem_free", "memcpy_from_symbol",
+                    "memcpy_to_symbol", "batch_mem_op")
+
+
+def census(fn, handle):
+    """Node counts of a captured graph by type name (nonzero types only)."""
+    counts = (ctypes.c_int32 * len(GRAPH_NODE_TYPES))()
+    check(fn(handle, counts, len(GRAPH_NODE_TYPES)))
+    return {name: int(c) for name, c in zip(GRAPH_NODE_TYPES, counts) if c}
 
 
 def check(code):

@@ -83,6 +83,10 @@ class BatchPeeler:
     def synchronize(self):
         _lib.check(self._lib.peel_batch_synchronize(self._handle))
 
+    def graph_nodes(self):
+        """Node counts of the batch graph by type (see DevicePeeler.graph_nodes)."""
+        return _lib.census(self._lib.peel_batch_graph_census, self._handle)
+
     def read_status(self):
         """Synchronise and decode every member's status (raises like the
         members' own read_status on the first failure)."""

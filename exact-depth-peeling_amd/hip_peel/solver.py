@@ -260,6 +260,14 @@ class _Workspace:
         _lib.check(self._lib.peel_set_tuning(self._handle, ctypes.byref(t)))
         self.graph_captured = False
 
+    def graph_nodes(self):
+        """Node counts of the captured graph by type (``kernel``, ``empty``,
+        ``event_record`` ...): the reference checks that its graph holds no
+        copies or allocations (tests/gpu/test_peeler.py:197-202)."""
+        if not self.graph_captured:
+            self.capture()
+        return _lib.census(self._lib.peel_graph_census, self._handle)
+
     def synchronize(self):
         _lib.check(self._lib.peel_synchronize(self._handle))
 

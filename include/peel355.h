@@ -141,6 +141,26 @@ int peel_get_tuning(PeelWorkspace *ws, PeelTuning *tuning);
  * created again. */
 int peel_set_tuning(PeelWorkspace *ws, const PeelTuning *tuning);
 
+/* Nodes of the captured graph by type: counts[t] = nodes of hipGraphNodeType
+ * t for t < n (kernel 0, memcpy 1, memset 2, host 3, empty 5, event wait 6,
+ * event record 7, alloc 10, free 11). Returns 9 before the first capture.
+ * It restates the reference's graph-content check (tests/gpu/test_peeler.py:
+ * 197-202: no device-to-host, host-to-device or allocation nodes) and
+ * counts the launches of one extraction. */
+int peel_graph_census(PeelWorkspace *ws, int32_t *counts, int32_t n);
+
+/* Unit-test entry into the device's ring functions, on one lane (no
+ * counterpart in the reference's API; it restates the reference's tests that
+ * launch its device functions directly, tests/gpu/test_peeler.py:331-375).
+ * op 0: clean (warp_peel/kernels.py:146-176); op 1: split_edge by the edge
+ * param = (ax, ay, bx, by) (kernels.py:206-212, outputs made strictly convex
+ * as the layered route makes them); op 2: the strict-convexity rule. Device
+ * pointers: xy [n <= 32][2], out_xy [2][32][2] (inside, outside), out_n [2];
+ * enqueued on `stream` without synchronising. Returns 0, 1 on bad arguments,
+ * -1 on a launch failure. */
+int peel_ring_test(int32_t op, const double *xy, int32_t n, const double *param, double *out_xy, int32_t *out_n,
+                   void *stream);
+
 /* Synchronise the workspace stream. */
 int peel_synchronize(PeelWorkspace *ws);
 
@@ -160,6 +180,8 @@ int peel_batch_create(PeelWorkspace *const *workspaces, int32_t count, PeelBatch
 int peel_batch_launch(PeelBatch *batch);
 int peel_batch_synchronize(PeelBatch *batch);
 int peel_batch_destroy(PeelBatch *batch);
+/* Node census of the batch graph, as peel_graph_census. */
+int peel_batch_graph_census(PeelBatch *batch, int32_t *counts, int32_t n);
 
 /* Learned pool sizes. No counterpart in the reference, whose capacities are
  * fixed multiples of the face count (warp_peel/events.py:19-33); it serves

@@ -121,3 +121,34 @@ def test_merged_batch_matches_single_peels():
     merged.run()
     again = merged.read_status()
     assert (first["polygons"], first["triangles"], first["layers"]) == (again["polygons"], again["triangles"], again["layers"])
+
+
+def test_batch_member_with_huge_hit_group():
+    """A member whose single ray group holds more than 2,048 hits (stacks of
+    5,000 and 3,000 identical footprints) is sorted one phase per launch in
+    the batch graph, where its blocks need not be co-resident; every member
+    equals its own peel and the stack's layers are its sources."""
+    from hip_peel import BatchPeeler, EventPeeler, scenes
+    from hip_peel.io import upload
+
+    print("[batch] huge hit groups", flush=True)
+    hosts = [scenes.stack(5000), scenes.stack(3000)] + [m[:4] for m in meshes(count=2, levels=(3, 3), seed=13)]
+    batch = BatchPeeler([EventPeeler(*upload(*h)) for h in hosts])
+    nodes = batch.graph_nodes()
+    assert set(nodes) <= {"kernel", "empty", "event_record", "wait_event"}, nodes
+    for _ in range(2):
+        batch.run()
+        infos = batch.read_status()
+        for k, (h, p, info) in enumerate(zip(hosts, batch.peelers, infos)):
+            single = EventPeeler(*upload(*h))
+            single.run()
+            one = single.read_status()
+            assert (info["layers"], info["polygons"], info["triangles"]) == (
+                one["layers"], one["polygons"], one["triangles"]), k
+            _, r1, s1, l1, *_ = gpu_outputs(p)
+            _, r2, s2, l2, *_ = gpu_outputs(single)
+            assert canonical_patches(r1, s1, l1) == canonical_patches(r2, s2, l2), k
+        for n, p in zip((5000, 3000), batch.peelers[:2]):
+            _, _, srcs, lays, *_ = gpu_outputs(p)
+            assert len(srcs) == n
+            np.testing.assert_array_equal(lays, srcs)

@@ -1,0 +1,197 @@
+"""The large workloads at their stated sizes under the oracle:
+
+* the 717,740-face nested shells (13 noisy closed spheres) through the event
+  route, binary64 and binary32-filtered: bit parity of every patch and
+  triangle with the CPU restatement, the exact rank at the interior points of
+  a seeded sample of fragments (>= 100,000 points), the grid audit, and area
+  conservation for every source;
+* the 256 uneven icosphere instances (levels 2-6, 6,116,480 faces) peeled as
+  one merged workspace: area conservation for every source of the atlas
+  scene, exact ranks at the interior points of a seeded fragment sample, the
+  grid audit on the atlas, and per-instance layer counts and areas equal to
+  single peels for a sample of instances.
+"""
+
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from peel_checks import grid_points, interior_points, source_areas
+
+pytestmark = pytest.mark.gpu
+
+
+def device_polygons(peeler, count):
+    """(rings (P, 32, 2) f64, n, sources, layers) of the valid prefix."""
+    xy = peeler._output_xy[:, :count].permute(1, 0, 2).contiguous().cpu().numpy()
+    n = peeler._output_n[:count].cpu().numpy()
+    return xy, n, peeler.output_sources[:count].cpu().numpy(), peeler.output_layers[:count].cpu().numpy()
+
+
+def device_areas(peeler, count, nsources):
+    """Per-source fragment areas, summed on the device (shoelace over the
+    valid vertices of each ring)."""
+    import torch
+
+    xy = peeler._output_xy[:, :count]  # (32, P, 2)
+    n = peeler._output_n[:count].long()
+    j = torch.arange(32, device=xy.device)[:, None]
+    nxt = (j + 1) % n.clamp(min=1)[None, :]
+    x1 = torch.gather(xy[..., 0], 0, nxt)
+    y1 = torch.gather(xy[..., 1], 0, nxt)
+    term = (xy[..., 0] * y1 - xy[..., 1] * x1) * (j < n[None, :])
+    area = 0.5 * term.sum(0)
+    out = torch.zeros(nsources, dtype=torch.float64, device=xy.device)
+    out.index_add_(0, peeler.output_sources[:count].long(), area)
+    return out.cpu().numpy(), area
+
+
+def row_digests(xy, n, src, layer):
+    """64-bit digest of each (source, layer, ring) record; multisets of
+    records compare as sorted digest arrays."""
+    P = len(n)
+    mask = np.arange(xy.shape[1])[None, :] < n[:, None]
+    words = np.where(mask[..., None], xy, 0.0).reshape(P, -1).view(np.uint64)
+    h = (src.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) ^ (layer.astype(np.uint64) << np.uint64(40))
+    h ^= n.astype(np.uint64)
+    for k in range(words.shape[1]):
+        h = (h ^ words[:, k]) * np.uint64(0x100000001B3)
+        h ^= h >> np.uint64(29)
+    return np.sort(h)
+
+
+def cpu_digests(cpu):
+    P = len(cpu.rings)
+    xy = np.zeros((P, 32, 2))
+    n = np.array([len(r) for r in cpu.rings], dtype=np.int32)
+    for i, r in enumerate(cpu.rings):
+        xy[i, : len(r)] = r
+    return row_digests(xy, n, cpu.sources, cpu.layers)
+
+
+def sampled_rank_check(host, xy, n, src, layer, fragments, seed=0):
+    rng = np.random.default_rng(seed)
+    live = np.nonzero(n > 0)[0]
+    pick = rng.choice(live, size=min(fragments, len(live)), replace=False)
+    rings = [xy[i, : n[i]] for i in pick]
+    pts, owner = interior_points(rings)
+    ranks = oracle.exact_ranks(*host, pts)
+    checked, bad = 0, []
+    for k in range(len(pts)):
+        if ranks.point_flags[k]:
+            continue
+        a, b = ranks.offsets[k], ranks.offsets[k + 1]
+        entries = dict(zip(ranks.sources[a:b].tolist(), ranks.ranks[a:b].tolist()))
+        f = pick[owner[k]]
+        checked += 1
+        if entries.get(int(src[f])) != int(layer[f]):
+            bad.append((int(f), int(src[f]), int(layer[f]), entries.get(int(src[f]))))
+    return checked, bad
+
+
+@pytest.mark.parametrize("filter_fp32", [False, True])
+def test_nested_shells_full_size(filter_fp32):
+    from hip_peel import EventPeeler, scenes
+    from hip_peel.io import upload
+
+    t0 = time.perf_counter()
+    host = scenes.nested_shells()
+    print(f"[large] nested shells {len(host[1])} faces, filter_fp32={filter_fp32}", flush=True)
+    p = EventPeeler(*upload(*host), filter_fp32=filter_fp32)
+    p.run()
+    info = p.read_status()
+    P = info["polygons"]
+    xy, n, src, layer = device_polygons(p, P)
+    # bit parity with the CPU restatement
+    cpu = oracle.cpu_event_peel(*host, threads=16)
+    assert cpu.counts["error"] == 0
+    for key in ("polygons", "triangles", "layers", "splits", "unique_contours", "contour_tests"):
+        assert info[key] == cpu.counts[key], (key, info[key], cpu.counts[key])
+    np.testing.assert_array_equal(row_digests(xy, n, src, layer), cpu_digests(cpu))
+    T = info["triangles"]
+    dev_tri = np.sort(np.concatenate([p.triangles_uv[:T].reshape(T, -1).cpu().numpy(),
+                                      p.triangles_world[:T].reshape(T, -1).cpu().numpy()], axis=1).view(np.uint64)
+                      .sum(axis=1, dtype=np.uint64))
+    cpu_tri = np.sort(np.concatenate([cpu.tri_uv.reshape(T, -1), cpu.tri_world.reshape(T, -1)], axis=1)
+                      .view(np.uint64).sum(axis=1, dtype=np.uint64))
+    np.testing.assert_array_equal(dev_tri, cpu_tri)
+    print(f"[large] parity ok ({time.perf_counter() - t0:.1f} s)", flush=True)
+    # every source's fragments tile its projection
+    got, _ = device_areas(p, P, len(host[1]))
+    np.testing.assert_allclose(got, source_areas(*host), atol=1e-9, rtol=0)
+    # exact ranks at >= 100,000 interior points of sampled fragments
+    checked, bad = sampled_rank_check(host, xy, n, src, layer, fragments=25000)
+    assert checked >= 100000 and not bad, (checked, bad[:5])
+    # grid audit
+    gp = grid_points(48)
+    audit = oracle.audit([xy[i, : n[i]] for i in range(P)], src, layer, gp, oracle.exact_ranks(*host, gp))
+    assert audit["mismatches"] == 0, audit
+    print(f"[large] nested shells: {P} fragments, {info['layers']} layers, {checked} exact points, "
+          f"{time.perf_counter() - t0:.1f} s", flush=True)
+
+
+def test_icospheres_256_merged_full_size():
+    import torch
+
+    from hip_peel import EventPeeler, MergedBatch, scenes
+    from hip_peel.batch import atlas_scene
+    from hip_peel.io import upload
+
+    t0 = time.perf_counter()
+    ms = scenes.icosphere_instances(256, seed=0)
+    faces = sum(len(m[1]) for m in ms)
+    print(f"[large] 256 icospheres, {faces} faces", flush=True)
+    assert faces == 6116480
+    merged = MergedBatch(ms, filter_fp32=True)
+    merged.run()
+    info = merged.read_status()
+    P = info["polygons"]
+    v, f, c, s, layout = atlas_scene([m[:4] for m in ms])
+    host = (v, f, c, s)
+    print(f"[large] merged: {P} fragments, {info['layers']} layers ({time.perf_counter() - t0:.1f} s)", flush=True)
+    # area conservation for every source of the atlas scene
+    got, _ = device_areas(merged.peeler, P, len(f))
+    np.testing.assert_allclose(got, source_areas(*host), atol=1e-9, rtol=0)
+    # exact ranks at interior points of a fragment sample, and the grid audit
+    xy, n, src, layer = device_polygons(merged.peeler, P)
+    checked, bad = sampled_rank_check(host, xy, n, src, layer, fragments=20000, seed=1)
+    assert checked >= 80000 and not bad, (checked, bad[:5])
+    # grid audit inside the tiles of a sample of instances (tiles are
+    # disjoint, so only an instance's own fragments can contain its points)
+    tiles = int(np.ceil(np.sqrt(len(ms))))
+    side = tiles * (1.0 + 1.0 / 64.0)
+    rng = np.random.default_rng(3)
+    audited = 0
+    for k in sorted(rng.choice(len(ms), size=4, replace=False).tolist()):
+        f0, f1, tx, ty = layout[k][1], layout[k + 1][1], layout[k][2], layout[k][3]
+        mine = np.nonzero((src >= f0) & (src < f1) & (n > 0))[0]
+        g = (np.arange(40) + 0.5) / 40
+        gx, gy = np.meshgrid(g, g)
+        pts = np.stack([(gx.ravel() + tx * (1 + 1 / 64)) / side, (gy.ravel() + ty * (1 + 1 / 64)) / side], axis=1)
+        audit = oracle.audit([xy[i, : n[i]] for i in mine], src[mine], layer[mine], pts,
+                             oracle.exact_ranks(*host, pts))
+        assert audit["mismatches"] == 0, (k, audit)
+        audited += audit["points_checked"]
+    assert audited > 1000
+    del xy
+    # per instance: layer count and per-source area equal a single peel's
+    rng = np.random.default_rng(2)
+    for k in sorted(rng.choice(len(ms), size=8, replace=False).tolist()):
+        m = ms[k]
+        single = EventPeeler(*upload(*m[:4]))
+        single.run()
+        one = single.read_status()
+        mine = merged.instance(k)
+        layers = int(mine["layers"].max()) + 1 if len(mine["layers"]) else 0
+        assert layers == one["layers"], (k, layers, one["layers"])
+        area, _ = device_areas(single, one["polygons"], len(m[1]))
+        mine_area = np.zeros(len(m[1]))
+        for r, sidx in zip(mine["rings"], mine["sources"]):
+            x, y = r[:, 0], r[:, 1]
+            mine_area[sidx] += 0.5 * float(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+        np.testing.assert_allclose(mine_area, area, rtol=0, atol=1e-9)
+        del single
+    torch.cuda.synchronize()
+    print(f"[large] icospheres: {checked} exact points, {time.perf_counter() - t0:.1f} s", flush=True)
