@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark: full all-layer peel on one MI355X, as graph replays.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME] [--extras]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
+                    [--no-large] [--no-extras] [--batch-graph] [--no-cpu]
 
 A step is one replay of the captured extraction graph (validation, welding,
 projection, topology, hierarchies, subdivision, ranks, noding, fans, lift) on
@@ -10,8 +11,10 @@ and readback are outside the timed region (the reference's protocol,
 scripts/benchmark_events.py:77-93). The headline workload is the 65,536-face
 (2,3) torus-knot tube, timed with binary32-filtered ray predicates (`value`)
 and with plain binary64 predicates; the 717,740-face nested shells are timed
-both ways as well. --extras adds the 65,536-layer identical-footprint stack
-and 256 uneven icospheres peeled by one batched graph launch.
+both ways as well. Extras (default on): the 65,536-layer identical-footprint
+stack, the 256 uneven icospheres peeled as one merged workspace, and the
+layered route (DevicePeeler) on the headline workload; --batch-graph adds the
+icospheres as one graph over 256 workspaces.
 
 Before timing, each workload's launch tuning is autotuned (hip_peel.tune;
 outputs do not depend on it; --no-tune keeps the defaults), as a user tunes a
@@ -19,7 +22,9 @@ workload once before replaying it.
 
 Prints one JSON line. ``roofline`` prices the dominant kernel's algorithmic
 bytes per launch against its own event-timed duration; ``cpu_baseline`` times
-the CPU FP64 restatement of the same route (oracle) on a bounded sample.
+the restatement of the reference's fast CPU method (fast-bvh-peel
+--resolved-mesh) on the same workload with the reference's protocol, and the
+CPU restatement of the device's own route as a second figure.
 """
 
 import argparse
@@ -35,6 +40,9 @@ for p in (str(ROOT), str(ROOT / "exact-depth-peeling_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+# the CPU baseline's OpenMP threads yield when idle, as in the reference's
+# protocol; set before any OpenMP runtime starts
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
 
 
 def parse():
@@ -43,11 +51,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="torus-knot-65536")
-    ap.add_argument("--extras", action="store_true", help="also time the stack and the batched icospheres")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 65,536-layer stack, the 256 merged icospheres and the layered route")
+    ap.add_argument("--batch-graph", action="store_true",
+                    help="also time the 256 icospheres as one graph over 256 workspaces")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-large", action="store_true", help="skip the 717,740-face nested shells")
     ap.add_argument("--no-tune", action="store_true", help="time the default launch tuning only")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument(
         "--copy-calibration",
         type=int,
@@ -228,6 +239,46 @@ def run_merged(steps, warmup, count=256, seed=0):
     return out
 
 
+def run_layered(name, steps):
+    """The layered route (DevicePeeler: whole-face ranks, then one clipping
+    pass per layer) timed as graph replays. Its per-layer cuts put more
+    distinct vertices on single polygon edges than the event route does; where
+    a workload exceeds the 128-vertex edge-noding bound (bit 128, the
+    reference's own maximum: the torus knot's long thin tube faces are cut
+    along the same edges many times) the overflow is reported and an
+    81,920-face noisy icosphere under a pinhole (a member of the batched
+    workload) is timed instead. The reference publishes 9.767 s for its 64K
+    dragon on an L40 (results/gpu/README.md:15)."""
+    import torch
+
+    from hip_peel import DevicePeeler, scenes
+    from hip_peel.io import upload
+
+    out = {"workload": name, "reference_64k_dragon_l40_ms": 9767.0}
+    candidates = [(name, scenes.CONFIGS[name]())]
+    candidates.append(("icosphere-81920", scenes.icosphere_instances(1, seed=0, levels=(6, 6))[0]))
+    for label, host in candidates:
+        p = DevicePeeler(*upload(*host, device="cuda:0"))
+        wall = time.perf_counter()
+        p.run()
+        status, _ = p._read()
+        first_s = time.perf_counter() - wall
+        if status[9]:
+            out[f"{label}_error_bits"] = int(status[9])
+            del p
+            continue
+        info = p.read_status()
+        timing = time_replays(p, steps if first_s < 5 else 1, 0)
+        out.update({"workload": label, "faces": len(host[1]), "median_ms": round(timing["median_ms"], 3),
+                    "layers": info["layers"], "polygons": info["polygons"], "triangles": info["triangles"],
+                    "passes": info["peeling_passes"], "peak_residual": info["peak_residual_patches"],
+                    "kernel_launches": p.graph_nodes().get("kernel")})
+        del p
+        break
+    torch.cuda.synchronize()
+    return out
+
+
 def learned_pools(name):
     """Pool memory of the learned sizes against the conservative capacities
     for one workload, with the overflow-rerun outcome and equal outputs."""
@@ -287,18 +338,66 @@ def measured_traffic(workload, kernel):
     return table.get(f"peel::{kernel}")
 
 
-def cpu_baseline(name, seconds):
-    """CPU FP64 restatement of the device route (oracle), timed on this host."""
+def cpu_fast_baseline(name, threads_all):
+    """The reference's fast CPU method (fast-bvh-peel --resolved-mesh),
+    restated in oracle/cpu_fast.cpp, timed by the reference's protocol
+    (scripts/benchmark_event_cpu.py:41-62): four extractions per thread count,
+    the first discarded, the median of the other three, OMP_WAIT_POLICY=PASSIVE
+    (set before the OpenMP runtime starts, at the top of this file). Scope
+    "extraction" is the reference's algorithm_seconds (classification, layers,
+    meshing; fast_peel.cpp:653-660); scope "full" adds projection and
+    topology."""
+    import statistics
+
+    import oracle
+    from hip_peel import scenes
+
+    host = scenes.CONFIGS[name]()
+    per = {}
+    result = None
+    for threads in sorted({1, 4, threads_all}):
+        result = oracle.fast_peel(*host, threads=threads, repeat=4, keep=False)
+        kept = result.extract_s[1:]
+        med = statistics.median(kept)
+        per[threads] = {"extraction_ms": round(1e3 * med, 3), "full_ms": round(1e3 * (med + result.setup_s), 3),
+                        "extraction_runs_ms": [round(1e3 * x, 3) for x in kept]}
+    cpu_max = None
     try:
-        import oracle
-        from hip_peel import scenes
-    except Exception as e:  # pragma: no cover
-        return {"value": None, "error": str(e)}
-    if not hasattr(oracle, "cpu_event_peel"):
-        return None
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            cpu_max = f.read().strip()
+    except OSError:
+        pass
+    return {
+        "value": per[threads_all]["extraction_ms"],
+        "unit": "ms",
+        "cores": threads_all,
+        "kind": "port",
+        "method": "fast-bvh-peel --resolved-mesh (src/fast_peel.cpp, peel_common.h), restated in oracle/cpu_fast.cpp",
+        "sample": f"4 extractions of {name} ({len(host[1])} faces) per thread count, first discarded, median of 3; "
+                  f"scope: extraction and meshing after projection/topology; host nproc={os.cpu_count()}, "
+                  f"cgroup cpu.max={cpu_max}, OMP_WAIT_POLICY={os.environ.get('OMP_WAIT_POLICY')}",
+        "by_threads": {str(k): v for k, v in per.items()},
+        "layers": result.counts["layers"],
+        "patches": result.counts["patches"],
+        "triangles": result.counts["faces"],
+    }
+
+
+def cpu_port_baseline(name, seconds):
+    """Secondary figure: the CPU FP64 restatement of the device's event route
+    (oracle/cpu_event.cpp), the same algorithm as the GPU path."""
+    import oracle
+    from hip_peel import scenes
+
     host = scenes.CONFIGS[name]()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return oracle.cpu_baseline(host, seconds=seconds, threads=threads, workload=name)
+    out = oracle.cpu_baseline(host, seconds=seconds, threads=threads, workload=name)
+    return {k: out[k] for k in ("value", "unit", "cores", "sample", "fewer_threads")}
+
+
+def progress(message):
+    """One line per finished stage on stderr (the JSON result is stdout's only line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {message}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -326,8 +425,10 @@ def main():
     p64, host, info64, stages64, timing64, _, _ = run_config(args.config, args.steps, args.warmup, tune=tune)
     tune64 = p64.tune_report
     del p64
+    progress(f"{args.config} binary64: {timing64['median_ms']:.4f} ms")
     peeler, host, info, stages, timing, total, nverts = run_config(args.config, args.steps, args.warmup,
                                                                    filter_fp32=True, host=host, tune=tune)
+    progress(f"{args.config} binary32-filtered: {timing['median_ms']:.4f} ms")
     # the patch and triangle sets are deterministic (ray counters depend on
     # which patch represents a component, so they vary between runs)
     for key in ("polygons", "triangles", "layers"):
@@ -390,6 +491,9 @@ def main():
                                           "target_visits", "contour_tests", "ray_candidate_tests")},
         "cpu_baseline": None,
     }
+    # one extraction's graph: kernel launches and the node types it holds
+    result["graph_nodes"] = peeler.graph_nodes()
+    result["launches_per_extraction"] = result["graph_nodes"].get("kernel", 0)
     del peeler
     if not args.no_large:
         name = "nested-shells-717740"
@@ -398,8 +502,10 @@ def main():
             p, h, i, st, t, tot, nv = run_config(name, max(3, args.steps // 2), 1, filter_fp32=flag, tune=tune)
             large[flag] = (i, t, st, p.tune_report)
             del p
+            progress(f"{name} filter_fp32={flag}: {t['median_ms']:.3f} ms")
         (i64, t64, st64, tr64), (i32, t32, st32, tr32) = large[False], large[True]
         learned = learned_pools(name)
+        progress(f"{name} learned pools: {learned}")
         d32 = i32["filtered_predicates"] + i32["fp64_fallbacks"]
         result["fp32_vs_fp64"][name] = {
             "fp32_filtered_ms": round(t32["median_ms"], 3),
@@ -414,7 +520,7 @@ def main():
             "tuning_fp32": tr32,
             "tuning_fp64": tr64,
         }
-    if args.extras:
+    if not args.no_extras:
         extra = {}
         for name in ("stack-65536",):
             p, h, i, st, t, tot, nv = run_config(name, max(3, args.steps // 2), 1)
@@ -422,11 +528,20 @@ def main():
                            "polygons": i["polygons"], "triangles": i["triangles"],
                            "stages_ms": {k: round(v, 3) for k, v in st.items() if k.startswith("k_")}}
             del p
-        extra["icospheres-256"] = run_batch(max(3, args.steps // 2), 1)
+            progress(f"{name}: {extra[name]['median_ms']} ms")
         extra["icospheres-256-merged"] = run_merged(max(3, args.steps // 2), 1)
+        progress(f"icospheres merged: {extra['icospheres-256-merged']['merged_ms']} ms")
+        if args.batch_graph:
+            extra["icospheres-256"] = run_batch(max(3, args.steps // 2), 1)
+        extra["layered-" + args.config] = run_layered(args.config, 3)
+        progress(f"layered route: {extra['layered-' + args.config]}")
         result["extra"] = extra
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(args.config, args.cpu_seconds)
+        threads_all = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        result["cpu_baseline"] = cpu_fast_baseline(args.config, threads_all)
+        progress(f"cpu fast baseline: {result['cpu_baseline']['value']} ms on {threads_all} threads")
+        result["cpu_baseline"]["event_route_port"] = cpu_port_baseline(args.config, args.cpu_seconds)
+        result["gpu_vs_cpu_fast"] = round(result["cpu_baseline"]["value"] / result["value"], 1)
     print(json.dumps(result))
 
 

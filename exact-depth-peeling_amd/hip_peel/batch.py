@@ -56,7 +56,8 @@ class BatchPeeler:
         if pools == "learned":
             model = model or pool_sizes.load_default()
             features = [pool_sizes.mesh_features(*m[:4]) for m in meshes]
-            caps = model.capacities_many(features, faces, device)
+            closed = [pool_sizes.closed_mesh(*m[:2]) for m in meshes]
+            caps = model.capacities_many(features, faces, device, closed)
         else:
             caps = [EventCapacities.for_mesh(n) for n in faces]
         peelers = [EventPeeler(*a, capacities=c, **options) for a, c in zip(arrays, caps)]
@@ -186,7 +187,8 @@ class MergedBatch:
         if pools == "learned":
             model = model or pool_sizes.load_default()
             features = [pool_sizes.mesh_features(*m) for m in self.meshes]
-            per = model.capacities_many(features, faces, device)
+            closed = [pool_sizes.closed_mesh(*m[:2]) for m in self.meshes]
+            per = model.capacities_many(features, faces, device, closed)
             fields = [p for p, _ in pool_sizes.TARGETS]
             cap = replace(cap, **{p: int(sum(getattr(c, p) for c in per)) for p in fields})
         elif pools != "conservative":

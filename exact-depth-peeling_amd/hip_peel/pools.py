@@ -61,6 +61,10 @@ TARGETS = (
 # workspace reports its exact size as ``device_bytes``)
 ELEMENT_BYTES = {"frontier": 532, "regions": 650, "hits": 40, "vertices": 110, "triangles": 128}
 MIN_CAPACITY = 64
+# Closed meshes of at least this many faces get a per-face floor on every
+# predicted pool (PoolModel.floor).
+FLOOR_MIN_FACES = 4096
+CLOSED_KINDS = ("icosphere", "knot")
 
 
 def mesh_features(vertices, faces, camera, settings):
@@ -131,6 +135,41 @@ def mesh_features(vertices, faces, camera, settings):
     return out
 
 
+def closed_mesh(vertices, faces):
+    """True if every undirected edge of the welded mesh has exactly two
+    incidences in opposite directions (a closed, consistently oriented
+    surface, which may have several components)."""
+    V = np.asarray(vertices, dtype=np.float64).reshape(-1, 3)
+    F = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
+    if len(F) == 0:
+        return False
+    _, canon = np.unique(V, axis=0, return_inverse=True)
+    fc = canon.reshape(-1)[F]
+    a, b = fc.reshape(-1), np.roll(fc, -1, axis=1).reshape(-1)
+    n = int(canon.max()) + 1
+    directed = a * n + b
+    undirected = np.minimum(a, b) * n + np.maximum(a, b)
+    _, counts = np.unique(undirected, return_counts=True)
+    if (counts != 2).any():
+        return False
+    return np.unique(directed).size == directed.size
+
+
+def closed_mesh_floor(usage, faces, kinds, quantile=0.995, min_faces=FLOOR_MIN_FACES):
+    """Per-face floor of each pool for closed meshes: the `quantile` of
+    use / faces over the logged runs of closed-surface generators with at least
+    `min_faces` faces. Their per-face use stays in a narrow band whatever the
+    depth complexity (each closed surface adds fragments along the contours
+    that cross it and one ray hit per covering surface), which lets unseen
+    closed meshes (such as nested shells) be admitted where the network,
+    trained on shallower scenes, under-predicts."""
+    usage, faces, kinds = np.asarray(usage, np.float64), np.asarray(faces), np.asarray(kinds)
+    pick = np.isin(kinds, CLOSED_KINDS) & (faces >= min_faces)
+    if not pick.any():
+        return np.zeros(usage.shape[1])
+    return np.quantile(usage[pick] / faces[pick][:, None], quantile, axis=0)
+
+
 def depth_complexity(tri, area, grid=DEPTH_GRID):
     """(covered fraction, log mean depth, log max depth) of projected faces:
     each face's area is spread evenly over the window cells its bounding box
@@ -167,18 +206,20 @@ def _device_call(code):
 class PoolModel:
     """Standardisation, network parameters and per-pool log margins."""
 
-    def __init__(self, params, x_mean, x_std, y_mean, y_std, margin, meta=None):
+    def __init__(self, params, x_mean, x_std, y_mean, y_std, margin, meta=None, floor=None):
         self.params = np.asarray(params, dtype=np.float32)
         self.x_mean, self.x_std = np.asarray(x_mean, np.float64), np.asarray(x_std, np.float64)
         self.y_mean, self.y_std = np.asarray(y_mean, np.float64), np.asarray(y_std, np.float64)
         self.margin = np.asarray(margin, dtype=np.float64)
         self.meta = dict(meta or {})
         self.inputs, self.outputs = len(self.x_mean), len(self.y_mean)
+        # per-face use floor of closed meshes (closed_mesh_floor), zeros = none
+        self.floor = np.zeros(self.outputs) if floor is None else np.asarray(floor, dtype=np.float64)
         self._device = {}
 
     def save(self, path):
         np.savez(path, params=self.params, x_mean=self.x_mean, x_std=self.x_std, y_mean=self.y_mean,
-                 y_std=self.y_std, margin=self.margin,
+                 y_std=self.y_std, margin=self.margin, floor=self.floor,
                  meta_keys=np.array(list(self.meta.keys())),
                  meta_values=np.array([float(v) for v in self.meta.values()]))
 
@@ -186,7 +227,8 @@ class PoolModel:
     def load(cls, path=MODEL_PATH):
         with np.load(path, allow_pickle=False) as z:
             meta = dict(zip(z["meta_keys"].tolist(), z["meta_values"].tolist()))
-            return cls(z["params"], z["x_mean"], z["x_std"], z["y_mean"], z["y_std"], z["margin"], meta)
+            floor = z["floor"] if "floor" in z.files else None
+            return cls(z["params"], z["x_mean"], z["x_std"], z["y_mean"], z["y_std"], z["margin"], meta, floor)
 
     def _params_on(self, device):
         import torch
@@ -212,15 +254,20 @@ class PoolModel:
                                           xt.data_ptr(), len(x), y.data_ptr(), stream.cuda_stream))
         return y.double().cpu().numpy() * self.y_std + self.y_mean + features[:, :1]
 
-    def capacities(self, features, faces, device="cuda:0"):
+    def capacities(self, features, faces, device="cuda:0", closed=False):
         """EventCapacities from the prediction plus the calibrated margin."""
-        return self.capacities_many(np.atleast_2d(features), [faces], device)[0]
+        return self.capacities_many(np.atleast_2d(features), [faces], device, [closed])[0]
 
-    def capacities_many(self, features, faces, device="cuda:0"):
-        """EventCapacities for each feature row (one device launch)."""
+    def capacities_many(self, features, faces, device="cuda:0", closed=None):
+        """EventCapacities for each feature row (one device launch); a closed
+        mesh of at least FLOOR_MIN_FACES faces gets at least `floor` per face
+        in every pool."""
         use = np.expm1(self.predict_log_usage(features, device) + self.margin)
+        closed = [False] * len(use) if closed is None else closed
         out = []
-        for row, n in zip(use, faces):
+        for row, n, shut in zip(use, faces, closed):
+            if shut and n >= FLOOR_MIN_FACES:
+                row = np.maximum(row, self.floor * n)
             caps = {pool: max(MIN_CAPACITY, int(np.ceil(u))) for (pool, _), u in zip(TARGETS, row)}
             out.append(replace(EventCapacities.for_mesh(int(n)), **caps))
         return out
@@ -432,7 +479,7 @@ def admitted_event_peeler(vertices, faces, camera, settings, model=None, **optio
     model = model or load_default()
     host = [t.detach().cpu().numpy() for t in (vertices, faces, camera, settings)]
     n = host[1].reshape(-1, 3).shape[0]
-    cap = model.capacities(mesh_features(*host), n, device=vertices.device)
+    cap = model.capacities(mesh_features(*host), n, device=vertices.device, closed=closed_mesh(*host[:2]))
     peeler = EventPeeler(vertices, faces, camera, settings, capacities=cap, **options)
     peeler.run()
     status, _ = peeler._read()

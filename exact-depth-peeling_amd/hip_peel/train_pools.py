@@ -12,7 +12,8 @@
    (``pools.PoolTrainer``: ``pool_mlp_gradient`` + ``pool_mlp_adam``) on
    log1p(use) - log(faces), with under-predictions weighted 4x;
 4. calibrates one log margin per pool on the calibration set
-   (``pools.calibrate_margin``);
+   (``pools.calibrate_margin``) and a per-face floor for closed meshes from
+   the closed-surface runs of the training set (``pools.closed_mesh_floor``);
 5. reports, on the test set, the fraction of runs that would overflow and the
    pool bytes against ``EventCapacities.for_mesh``, and writes the model
    (``hip_peel/pool_model.npz``) and the logged runs
@@ -46,7 +47,7 @@ def conservative(faces):
     return np.array(rows, dtype=np.float64)
 
 
-def train(features, usage, faces, steps, seed=0, lr=3e-3, under=4.0, device="cuda:0"):
+def train(features, usage, faces, steps, seed=0, lr=3e-3, under=4.0, device="cuda:0", kinds=None):
     """Returns (model, report)."""
     n = len(features)
     rng = np.random.default_rng(seed)
@@ -63,7 +64,8 @@ def train(features, usage, faces, steps, seed=0, lr=3e-3, under=4.0, device="cud
                         under=under, log_every=max(1, steps // 20))
     seconds = time.perf_counter() - t0
     params = trainer.params.cpu().numpy()
-    model = pools.PoolModel(params, x_mean, x_std, y_mean, y_std, np.zeros(y.shape[1]))
+    floor = pools.closed_mesh_floor(usage[tr], faces[tr], kinds[tr]) if kinds is not None else None
+    model = pools.PoolModel(params, x_mean, x_std, y_mean, y_std, np.zeros(y.shape[1]), floor=floor)
     margin = pools.calibrate_margin(y[cal], model.predict_log_usage(features[cal], device))
     model.margin = margin
     pred = model.predict_log_usage(features[te], device)
@@ -111,7 +113,7 @@ def main(argv=None):
                                                            workers=args.workers)
         print(f"collected {len(features)} runs in {time.perf_counter() - t0:.1f} s", flush=True)
         np.savez_compressed(args.runs_out, features=features, usage=usage, kinds=kinds, faces=faces)
-    model, report = train(features, usage, faces, args.steps, seed=args.seed, device=args.device)
+    model, report = train(features, usage, faces, args.steps, seed=args.seed, device=args.device, kinds=kinds)
     model.save(args.out)
     report["kinds"] = {str(k): int((kinds == k).sum()) for k in np.unique(kinds)}
     print(json.dumps(report))

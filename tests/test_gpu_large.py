@@ -195,3 +195,24 @@ def test_icospheres_256_merged_full_size():
         del single
     torch.cuda.synchronize()
     print(f"[large] icospheres: {checked} exact points, {time.perf_counter() - t0:.1f} s", flush=True)
+
+
+def test_nested_shells_admitted_by_learned_pools():
+    """The nested shells are unlike any mesh of the training generators; the
+    closed-mesh floor admits them at the learned sizes without a rerun, with
+    the conservative run's outputs."""
+    from hip_peel import EventPeeler, pools, scenes
+    from hip_peel.io import upload
+
+    arrays = upload(*scenes.nested_shells())
+    base = EventPeeler(*arrays)
+    base.run()
+    want = base.read_status()
+    peeler, rebuilt = pools.admitted_event_peeler(*arrays)
+    got = peeler.read_status()
+    print(f"[large] learned pools: {peeler.device_bytes / 2**30:.2f} GiB vs conservative "
+          f"{base.device_bytes / 2**30:.2f} GiB, rebuilt={rebuilt}", flush=True)
+    assert not rebuilt
+    for key in ("polygons", "triangles", "layers", "splits", "unique_contours"):
+        assert got[key] == want[key], key
+    assert peeler.device_bytes < 0.5 * base.device_bytes

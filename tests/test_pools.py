@@ -141,3 +141,46 @@ def test_depth_complexity_closed_form():
     assert x[names.index("log_depth_max")] == pytest.approx(np.log1p(2.0))
     x = pools.mesh_features(v, f[:1], np.eye(4), np.array([0.0, 0.0, 1.0, 1.0, 0.0, 0.01]))
     assert x[names.index("log_depth_mean")] == pytest.approx(np.log1p(0.5))
+
+
+def test_closed_mesh_detection():
+    from hip_peel import pools, scenes
+
+    v, f = scenes.geodesic_sphere(1.0, 4)
+    assert pools.closed_mesh(v, f)
+    # two separate closed surfaces are still closed
+    assert pools.closed_mesh(np.concatenate([v, v + 5.0]), np.concatenate([f, f + len(v)]))
+    assert not pools.closed_mesh(v, f[1:])  # a hole
+    flipped = f.copy()
+    flipped[0] = flipped[0][::-1]
+    assert not pools.closed_mesh(v, flipped)  # inconsistent orientation
+    assert not pools.closed_mesh(*scenes.stack(8)[:2])  # unwelded soup
+    # an unwelded copy of a closed mesh welds back to it
+    assert pools.closed_mesh(v[f].reshape(-1, 3), np.arange(3 * len(f)).reshape(-1, 3))
+
+
+def test_closed_mesh_floor_and_capacities():
+    """The floor is the per-face use quantile of the large closed-surface
+    runs, and only closed meshes of at least FLOOR_MIN_FACES faces get it."""
+    from hip_peel import pools
+
+    usage = np.array([[10, 20, 30, 40, 50], [100, 100, 100, 100, 100], [1, 1, 1, 1, 1], [9e5] * 5], float)
+    faces = np.array([5000, 10000, 100, 6000])
+    kinds = np.array(["icosphere", "knot", "icosphere", "soup"])
+    floor = pools.closed_mesh_floor(usage, faces, kinds, quantile=1.0)
+    # per-face use [0.002 .. 0.01] and 0.01: soups and meshes below 4,096 faces are excluded
+    np.testing.assert_allclose(floor, [0.01] * 5)
+    m = pools.load_default()
+    assert (m.floor > 0).all()
+
+
+def test_nested_shells_floor_covers_recorded_use():
+    """The committed model's floor admits the 717,740-face nested shells, a
+    closed mesh none of the training generators produce: their recorded use
+    per face (frontier 228,226 slots, 827,321 polygons, 1,353,120 ray hits,
+    2,734,657 triangles) lies below it."""
+    from hip_peel import pools
+
+    m = pools.load_default()
+    used = np.array([228226, 827321, 1353120, 2734657]) / 717740
+    np.testing.assert_array_less(used, m.floor[[0, 1, 2, 4]])
