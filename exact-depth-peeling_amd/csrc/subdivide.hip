@@ -59,6 +59,8 @@ __device__ __forceinline__ double2 vtx(const Slot &s, int j) { return s.xy[(long
 constexpr int SUB_THREADS = 256;
 constexpr int SUB_LDS = 12;
 __shared__ double2 sub_ring_lds[SUB_LDS * SUB_THREADS];
+// each lane's hierarchy walk stack (WideWalk), entry k at [k * SUB_THREADS + lane]
+__shared__ int sub_walk_lds[WALK_STACK * SUB_THREADS];
 // kept out of line so the compiler does not fold the LDS and global loads
 // into one generic load
 __device__ __noinline__ double2 slot_vertex(const double2 *slot, long stride, int j) {
@@ -112,6 +114,7 @@ __device__ void save_output(const Ws &w, const LdsRing &s, int n, int source, in
 
 __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
   LdsRing ring{(int)threadIdx.x, nullptr, 0};
+  WideWalk walk{sub_walk_lds + threadIdx.x, SUB_THREADS, 0};
   const int nf = w.nf;
   const Tree tree = w.contour_tree.tree();
   int lane = threadIdx.x & 63;
@@ -162,6 +165,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
           for (int j = 0; j < n && j < SUB_LDS; ++j) ring.put(j, vtx(src, j));
           ring.slot = src.xy;
           ring.stride = src.stride;
+          walk.start(node);
         }
       }
     }
@@ -198,11 +202,14 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
     lo.z = -1e30f;
     bool done = false, cut = false;
     for (int step = 0; step < 32; ++step) {
-      int c = query_next(tree, lo, hi, node);
+      int leaf = 0;
+      int c = walk.next(tree, lo, hi, leaf);
       if (c < 0) {
+        node = 0;
         done = true;
         break;
       }
+      node = successor(leaf);
       tests += 1;
       double2 a = w.cseg_a[c], b = w.cseg_b[c];
       bool positive = false, negative = false;
