@@ -103,6 +103,31 @@ def source_areas(vertices, faces, camera, settings):
     return areas
 
 
+CLEANUP_WIDTH = 1e-10  # warp_peel/README.md:138-141: remnants narrower than ~1e-10 are discarded
+
+
+def source_edge_distance(vertices, faces, camera, settings, points, pad=1e-6):
+    """Distance (normalised window units) from each point to the nearest
+    projected edge of any source of an orthographic scene without near
+    clipping; only sources whose projected box, padded by `pad`, holds the
+    point are examined (a nearer edge of a farther source would need a
+    distance above `pad`, where this returns `pad`)."""
+    assert not settings[4]
+    cam = vertices[faces] @ camera[:3, :3].T + camera[:3, 3]
+    uv = (cam[..., :2] - settings[:2]) / (settings[2:4] - settings[:2])
+    lo, hi = uv.min(axis=1) - pad, uv.max(axis=1) + pad
+    out = np.full(len(points), pad)
+    for k, p in enumerate(np.asarray(points, dtype=np.float64).reshape(-1, 2)):
+        near = np.nonzero(np.all(lo <= p, axis=1) & np.all(hi >= p, axis=1))[0]
+        for f in near:
+            for j in range(3):
+                a, b = uv[f, j], uv[f, (j + 1) % 3]
+                d = b - a
+                t = np.clip(np.dot(p - a, d) / max(np.dot(d, d), 1e-300), 0.0, 1.0)
+                out[k] = min(out[k], float(np.hypot(*(p - (a + t * d)))))
+    return out
+
+
 def fragments(peeler, info=None):
     info = info or peeler.read_status()
     count = info["polygons"]

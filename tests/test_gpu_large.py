@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import oracle
-from peel_checks import grid_points, interior_points, source_areas
+from peel_checks import CLEANUP_WIDTH, grid_points, interior_points, source_areas, source_edge_distance
 
 pytestmark = pytest.mark.gpu
 
@@ -71,14 +71,18 @@ def cpu_digests(cpu):
     return row_digests(xy, n, cpu.sources, cpu.layers)
 
 
-def sampled_rank_check(host, xy, n, src, layer, fragments, seed=0):
+def sampled_rank_check(host, xy, n, src, layer, fragments, seed=0, band=0.0):
+    """Exact rank of each sampled fragment's source at its interior points.
+    With `band` > 0, a disagreeing point closer than `band` to a projected
+    source edge is returned in `banded` instead of `bad` (the route's cleanup
+    width: a remnant narrower than that may be merged into its neighbour)."""
     rng = np.random.default_rng(seed)
     live = np.nonzero(n > 0)[0]
     pick = rng.choice(live, size=min(fragments, len(live)), replace=False)
     rings = [xy[i, : n[i]] for i in pick]
     pts, owner = interior_points(rings)
     ranks = oracle.exact_ranks(*host, pts)
-    checked, bad = 0, []
+    checked, bad, banded = 0, [], []
     for k in range(len(pts)):
         if ranks.point_flags[k]:
             continue
@@ -87,8 +91,12 @@ def sampled_rank_check(host, xy, n, src, layer, fragments, seed=0):
         f = pick[owner[k]]
         checked += 1
         if entries.get(int(src[f])) != int(layer[f]):
-            bad.append((int(f), int(src[f]), int(layer[f]), entries.get(int(src[f]))))
-    return checked, bad
+            bad.append((int(f), int(src[f]), int(layer[f]), entries.get(int(src[f])), k))
+    if band > 0 and bad:
+        dist = source_edge_distance(*host, pts[[e[-1] for e in bad]])
+        banded = [e + (float(d),) for e, d in zip(bad, dist) if d < band]
+        bad = [e + (float(d),) for e, d in zip(bad, dist) if d >= band]
+    return checked, bad, banded
 
 
 @pytest.mark.parametrize("filter_fp32", [False, True])
@@ -122,7 +130,7 @@ def test_nested_shells_full_size(filter_fp32):
     got, _ = device_areas(p, P, len(host[1]))
     np.testing.assert_allclose(got, source_areas(*host), atol=1e-9, rtol=0)
     # exact ranks at >= 100,000 interior points of sampled fragments
-    checked, bad = sampled_rank_check(host, xy, n, src, layer, fragments=25000)
+    checked, bad, _ = sampled_rank_check(host, xy, n, src, layer, fragments=25000)
     assert checked >= 100000 and not bad, (checked, bad[:5])
     # grid audit
     gp = grid_points(48)
@@ -156,8 +164,15 @@ def test_icospheres_256_merged_full_size():
     np.testing.assert_allclose(got, source_areas(*host), atol=1e-9, rtol=0)
     # exact ranks at interior points of a fragment sample, and the grid audit
     xy, n, src, layer = device_polygons(merged.peeler, P)
-    checked, bad = sampled_rank_check(host, xy, n, src, layer, fragments=20000, seed=1)
-    assert checked >= 80000 and not bad, (checked, bad[:5])
+    # the atlas window is 16.25 instance windows wide, so the route's cleanup
+    # width (1e-10 of the window) is 1.6e-9 of an instance's: a point that
+    # close to a source edge may lie in a merged remnant (on the CPU
+    # restatement, instance 145 has two such points, 4.8e-11 from one edge of a
+    # near-collinear silhouette fan; peeled alone it has none)
+    checked, bad, banded = sampled_rank_check(host, xy, n, src, layer, fragments=20000, seed=1, band=CLEANUP_WIDTH)
+    print(f"[large] merged: {checked} exact points, {len(banded)} within the cleanup width of a source edge "
+          f"{banded[:3]}", flush=True)
+    assert checked >= 80000 and not bad and len(banded) <= 1e-4 * checked, (checked, bad[:5], banded[:5])
     # grid audit inside the tiles of a sample of instances (tiles are
     # disjoint, so only an instance's own fragments can contain its points)
     tiles = int(np.ceil(np.sqrt(len(ms))))

@@ -248,3 +248,19 @@ def test_audit_boundary_distance():
     gp = grid_points(48)
     report = oracle.audit(cpu.rings, cpu.sources, cpu.layers, gp, oracle.exact_ranks(*host, gp))
     assert report["mismatches"] == 0 and report["pairs_on_fragment_boundary"] > 0
+
+
+def test_source_edge_distance():
+    """The cleanup-band helper of the merged-batch check: distances to the
+    nearest projected edge, in normalised window units, against hand values."""
+    from peel_checks import source_edge_distance
+
+    V = np.array([[0.0, 0.0, 1.0], [2.0, 0.0, 1.0], [0.0, 2.0, 1.0], [3.0, 3.0, 2.0]])
+    F = np.array([[0, 1, 2], [1, 3, 2]])
+    S = np.array([0.0, 0.0, 4.0, 4.0, 0.0, 0.1])
+    pts = np.array([[0.25, 0.1], [0.1, 0.1], [0.25, 0.25], [0.9, 0.9]])
+    d = source_edge_distance(V, F, np.eye(4), S, pts, pad=0.2)
+    # (0.25, 0.1): 0.1 above the edge y = 0; (0.1, 0.1): 0.1 from two edges;
+    # (0.25, 0.25): on the shared edge x + y = 0.5; (0.9, 0.9): 0.21 from the
+    # nearest vertex, beyond the pad
+    np.testing.assert_allclose(d, [0.1, 0.1, 0.0, 0.2], atol=1e-15)
