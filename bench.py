@@ -239,6 +239,17 @@ def run_merged(steps, warmup, count=256, seed=0):
     return out
 
 
+def filter_classes(info):
+    """Binary32 decisions and binary64 evaluations per filtered predicate class."""
+    from hip_peel.events import FILTER_CLASSES
+
+    out = {}
+    for c in FILTER_CLASSES:
+        f, b = info[f"filtered_{c}"], info[f"fp64_{c}"]
+        out[c] = {"filtered": f, "fp64": b, "fallback_fraction": round(b / (f + b), 6) if f + b else None}
+    return out
+
+
 def run_layered(name, steps):
     """The layered route (DevicePeeler: whole-face ranks, then one clipping
     pass per layer) timed as graph replays. Its per-layer cuts put more
@@ -483,6 +494,8 @@ def main():
                 "filtered_predicates": info["filtered_predicates"],
                 "fp64_fallbacks": info["fp64_fallbacks"],
                 "fallback_fraction": round(info["fp64_fallbacks"] / decided, 6) if decided else None,
+                "by_class": filter_classes(info),
+                "fp32_stages_ms": {k: round(v, 4) for k, v in stages.items() if k.startswith("k_")},
                 "fp64_stages_ms": {k: round(v, 4) for k, v in stages64.items() if k.startswith("k_")},
             }
         },
@@ -515,7 +528,9 @@ def main():
             "polygons": i32["polygons"],
             "triangles": i32["triangles"],
             "fallback_fraction": round(i32["fp64_fallbacks"] / d32, 6) if d32 else None,
+            "by_class": filter_classes(i32),
             "stages_ms": {k: round(v, 3) for k, v in st32.items() if k.startswith("k_")},
+            "fp64_stages_ms": {k: round(v, 3) for k, v in st64.items() if k.startswith("k_")},
             "learned_pools": learned,
             "tuning_fp32": tr32,
             "tuning_fp64": tr64,

@@ -174,6 +174,7 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
   c.take(w.leaf_score, w.source_tree.size);
   c.take(w.leaf_xy32, w.filter32 ? 8L * w.source_tree.size : 1);
   c.take(w.leaf_tol, w.filter32 ? w.source_tree.size : 1);
+  c.take(w.scores32, w.filter32 ? nf1 : 1);
   // layered route: residual pools sized by the frontier capacity, per-worker
   // piece pools, per-layer statistics
   const long RES = layered ? w.cap.frontier : 1, NF = layered ? nf1 : 1;

@@ -74,12 +74,18 @@ enum Metric : int {
   M_MERGED = 16,
   M_MATCHED = 17,
   M_HASH_MISSES = 18,
-  M_FILTER_TESTS = 19,   // predicates decided by the binary32 filter
-  M_FALLBACKS = 20,      // predicates that needed binary64
+  M_FILTER_TESTS = 19,   // ray containment edges decided by the binary32 filter
+  M_FALLBACKS = 20,      // ray containment edges that needed binary64
   M_SLOTS = 21,          // frontier slots used (seeds + 2 per split)
   M_EDGE_MAX = 22,       // most noding candidates on one polygon edge
   M_UNIQUE_BVERTS = 23,  // distinct (layer, point) boundary vertices
-  M_COUNT = 24,
+  M_FILTER_STRADDLE = 24,    // subdivision straddle signs decided in binary32
+  M_FALLBACK_STRADDLE = 25,  // ... evaluated in binary64
+  M_FILTER_FRONT = 26,       // front_segment edge tests decided in binary32
+  M_FALLBACK_FRONT = 27,     // ... evaluated in binary64
+  M_FILTER_DEPTH = 28,       // rank depth comparisons decided in binary32
+  M_FALLBACK_DEPTH = 29,     // ... evaluated in binary64
+  M_COUNT = 30,
 };
 
 // Control words (device) used by persistent kernels and on-device counts.
@@ -183,6 +189,39 @@ __device__ __forceinline__ float orientation_tol32(double D, double M) {
   const double u = 5.9604644775390625e-8;  // 2^-24
   double t = 16.0 * u * D * (M + D) + 1.0e-37;
   return __double2float_ru(t);
+}
+
+// The same orientation in binary32 on rounded inputs (no contraction: every
+// HIP unit is built with -ffp-contract=off).
+__device__ __forceinline__ float cross32(float2 a, float2 b, float2 c) {
+  return (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
+}
+__device__ __forceinline__ float2 to32(double2 x) { return make_float2((float)x.x, (float)x.y); }
+
+// Bounds of a subdivision candidate test: every point it touches (the target
+// ring inside its box [lx, hx] x [ly, hy], the segment a-b) lies in the union
+// box, so D is that box's larger side and M its largest coordinate magnitude.
+__device__ __forceinline__ float candidate_tol32(double lx, double ly, double hx, double hy, double2 a,
+                                                 double2 b) {
+  double x0 = fmin(lx, fmin(a.x, b.x)), x1 = fmax(hx, fmax(a.x, b.x));
+  double y0 = fmin(ly, fmin(a.y, b.y)), y1 = fmax(hy, fmax(a.y, b.y));
+  double D = fmax(x1 - x0, y1 - y0);
+  double M = fmax(fmax(fabs(x0), fabs(x1)), fmax(fabs(y0), fabs(y1)));
+  return orientation_tol32(D, M);
+}
+
+// Depth of plane s at x, s.x x.x + s.y x.y + s.z, in binary32 on rounded
+// inputs, with A = |s.x||x.x| + |s.y||x.y| + |s.z|: each of the five
+// roundings (two inputs and the product of each term, the two sums) is at
+// most 2^-24 relative, so the value is within 5.01 2^-24 A of the exact depth,
+// and the binary64 value within 3.4e-16 A. Two depths whose binary32
+// difference exceeds DEPTH_TOL32 (A1 + A2) (three times the sum of both
+// errors, which also covers the difference's own rounding and the rounding
+// of A) therefore compare in binary64 as they do in binary32, and unequal.
+constexpr float DEPTH_TOL32 = 16.0f * 5.9604644775390625e-8f;
+__device__ __forceinline__ float value32(float4 s, float2 x, float &A) {
+  A = fabsf(s.x) * fabsf(x.x) + fabsf(s.y) * fabsf(x.y) + fabsf(s.z);
+  return s.x * x.x + s.y * x.y + s.z;
 }
 
 // Kernels over pools whose live count is known only on the device (polygons,

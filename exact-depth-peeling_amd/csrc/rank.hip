@@ -304,6 +304,10 @@ __global__ void k_leaf_sources(Ws w) {
   }
   w.leaf_n[k] = n;
   if (w.filter32) {
+    if (id < w.nf) {
+      double4 sc = w.scores[id];
+      w.scores32[id] = make_float4((float)sc.x, (float)sc.y, (float)sc.z, 0.f);
+    }
     // binary32 copy of the ring and the static forward error bound of its
     // orientation tests against any point a ray query can bring to this leaf
     double lx = 1e300, ly = 1e300, hx = -1e300, hy = -1e300;
@@ -946,30 +950,55 @@ __global__ __launch_bounds__(HUGE_THREADS) void k_sort_huge_phase(Ws w, int phas
   }
 }
 
-__device__ int assign_one(const Ws &w, int i);
+__device__ int assign_one(const Ws &w, int i, int &filtered, int &fallbacks);
 
 __global__ __launch_bounds__(256) void k_assign(Ws w) {
-  int top = 0;
+  int top = 0, filtered = 0, fallbacks = 0;
   if (w.status[ST_ERR] == 0) {
-    ITEM_LOOP(i, npolys(w)) top = max(top, assign_one(w, i));
+    ITEM_LOOP(i, npolys(w)) top = max(top, assign_one(w, i, filtered, fallbacks));
   }
   top = wave_max(top);
   if ((threadIdx.x & 63) == 0 && top > 0) atomicMax(w.status + ST_LAYERS, top);
+  if (w.filter32) {
+    wave_add_metric(&w.metrics[M_FILTER_DEPTH], filtered);
+    wave_add_metric(&w.metrics[M_FALLBACK_DEPTH], fallbacks);
+  }
 }
 
-__device__ int assign_one(const Ws &w, int i) {
+__device__ int assign_one(const Ws &w, int i, int &filtered, int &fallbacks) {
   int source = w.out_src[w.reps[i]];
   double2 x = w.samples[i];
   double z = value(w.scores[source], x);
   int g = w.groups[i];
   int low = 0, high = w.group_counts[g];
   int base = w.group_offsets[g];
+  float2 x32 = to32(x);
+  float Az = 0.f, z32 = 0.f;
+  if (w.filter32) z32 = value32(w.scores32[source], x32, Az);
   while (low < high) {
     int mid = (low + high) / 2;
     int h = w.hids[base + mid];
     int hs = w.hit_src[h];
-    double hz = value(w.scores[hs], x);
-    if (hz < z || (hz == z && hs < source))
+    // binary32 depth order beyond its static bound; binary64 for the rest
+    int decided = 0;  // -1 nearer, +1 farther, 0 undecided
+    if (w.filter32 && hs == source) {
+      decided = 1;  // the patch's own hit: equal depth, and the tie rule says not nearer
+    } else if (w.filter32) {
+      float Ah;
+      float diff = value32(w.scores32[hs], x32, Ah) - z32;
+      float tol = DEPTH_TOL32 * (Ah + Az) + 1.0e-37f;
+      decided = diff < -tol ? -1 : (diff > tol ? 1 : 0);
+      if (decided)
+        filtered += 1;
+      else
+        fallbacks += 1;
+    }
+    bool nearer = decided < 0;
+    if (decided == 0) {
+      double hz = value(w.scores[hs], x);
+      nearer = hz < z || (hz == z && hs < source);
+    }
+    if (nearer)
       low = mid + 1;
     else
       high = mid;

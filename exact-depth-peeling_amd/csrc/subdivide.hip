@@ -94,6 +94,44 @@ __device__ bool front_segment(const LdsRing &s, int n, double2 a, double2 b, dou
   return high > low;
 }
 
+// front_segment with a binary32 filter on its ring-edge tests: an edge whose
+// two binary32 orientations are both beyond `tol` on the negative side
+// rejects (binary64 would reject there or earlier), both beyond it on the
+// positive side leaves the interval as binary64 would; only the other edges
+// are evaluated in binary64, where the interval needs their values. The result
+// therefore equals front_segment's.
+__device__ bool front_segment32(const LdsRing &s, int n, double2 a, double2 b, double za, double zb, float tol,
+                                int &filtered, int &fallbacks) {
+  const float2 a32 = to32(a), b32 = to32(b);
+  double low = 0.0, high = 1.0;
+  double2 u = s[0];
+  float2 u32 = to32(u);
+  for (int j = 0; j < n; ++j) {
+    double2 v = s[(j + 1) % n];
+    float2 v32 = to32(v);
+    float da32 = cross32(u32, v32, a32), db32 = cross32(u32, v32, b32);
+    if (da32 < -tol && db32 < -tol) {
+      filtered += 1;
+      return false;
+    }
+    if (da32 > tol && db32 > tol) {
+      filtered += 1;
+    } else {
+      fallbacks += 1;
+      double da = cross3(u, v, a), db = cross3(u, v, b);
+      if (da <= 0.0 && db <= 0.0) return false;
+      if (da < 0.0) low = fmax(low, da / (da - db));
+      if (db < 0.0) high = fmin(high, da / (da - db));
+    }
+    u = v;
+    u32 = v32;
+  }
+  if (za <= 0.0 && zb <= 0.0) return false;
+  if (za < 0.0) low = fmax(low, za / (za - zb));
+  if (zb < 0.0) high = fmin(high, za / (za - zb));
+  return high > low;
+}
+
 __device__ __forceinline__ int append_slot(double2 *xy, int stride, int n, double2 x) {
   if (n < MAXV) xy[(long)n * stride] = x;
   return n + 1;
@@ -122,6 +160,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
   int task = -1, ticket = -1;
   int n = 0, source = 0, node = 1, gen = 0;
   long long visits = 0, tests = 0, splits = 0;
+  int f_straddle = 0, b_straddle = 0, f_front = 0, b_front = 0;
   int max_gen = 0;
   uint64_t idle_since = wall_clock64();
   while (true) {
@@ -213,15 +252,41 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
       tests += 1;
       double2 a = w.cseg_a[c], b = w.cseg_b[c];
       bool positive = false, negative = false;
-      for (int j = 0; j < n; ++j) {
-        double side = cross3(a, b, s[j]);
-        positive |= side > 0.0;
-        negative |= side < 0.0;
+      float tol = 0.f;
+      if (w.filter32) {
+        // binary32 straddle signs beyond the candidate's static bound, the
+        // rest in binary64 (same signs, so the same decision)
+        tol = candidate_tol32(lx, ly, hx, hy, a, b);
+        const float2 a32 = to32(a), b32 = to32(b);
+        for (int j = 0; j < n; ++j) {
+          double2 p = s[j];
+          float side32 = cross32(a32, b32, to32(p));
+          if (side32 > tol) {
+            positive = true;
+            f_straddle += 1;
+          } else if (side32 < -tol) {
+            negative = true;
+            f_straddle += 1;
+          } else {
+            b_straddle += 1;
+            double side = cross3(a, b, p);
+            positive |= side > 0.0;
+            negative |= side < 0.0;
+          }
+        }
+      } else {
+        for (int j = 0; j < n; ++j) {
+          double side = cross3(a, b, s[j]);
+          positive |= side > 0.0;
+          negative |= side < 0.0;
+        }
       }
       if (!(positive && negative)) continue;
       double za = value(score, a) - w.cnear[c];
       double zb = value(score, b) - w.cnear[w.nseg + c];
-      if (!front_segment(s, n, a, b, za, zb)) continue;
+      if (w.filter32 ? !front_segment32(s, n, a, b, za, zb, tol, f_front, b_front)
+                     : !front_segment(s, n, a, b, za, zb))
+        continue;
       // ---- cut by the supporting line of (a, b), relative to a
       int k = atomicAdd(w.control + C_ALLOC, 2);
       if (k + 2 > w.cap.frontier) {
@@ -294,6 +359,12 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
   wave_add_metric(&w.metrics[M_VISITS], visits);
   wave_add_metric(&w.metrics[M_CONTOUR_TESTS], tests);
   wave_add_metric(&w.metrics[M_SPLITS], splits);
+  if (w.filter32) {
+    wave_add_metric(&w.metrics[M_FILTER_STRADDLE], f_straddle);
+    wave_add_metric(&w.metrics[M_FALLBACK_STRADDLE], b_straddle);
+    wave_add_metric(&w.metrics[M_FILTER_FRONT], f_front);
+    wave_add_metric(&w.metrics[M_FALLBACK_FRONT], b_front);
+  }
   max_gen = wave_max(max_gen);
   if ((threadIdx.x & 63) == 0)
     atomicMax((unsigned long long *)&w.metrics[M_ROUNDS], (unsigned long long)(max_gen + 1));

@@ -70,6 +70,7 @@ struct Ws {
   int filter32;            // binary32-filtered orientation tests (PEEL_FILTER_FP32)
   float2 *leaf_xy32;       // [source_tree.size][8] binary32 copies of the rings
   float *leaf_tol;         // [source_tree.size] their static error bounds
+  float4 *scores32;        // [nf] binary32 copies of the depth planes (filtered depth order)
 
   // frontier pool [MAXV][frontier]
   double2 *fr_xy;

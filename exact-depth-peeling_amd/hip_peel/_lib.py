@@ -12,7 +12,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libpeel355.so"
 ROUTE_EVENTS, ROUTE_LAYERED = 0, 1
 SHARE_REGIONS, PROPAGATE_RANKS, MERGE_REGIONS, FILTER_FP32 = 1, 2, 4, 8
 MAX_VERTICES = 32
-METRICS_COUNT = 24
+METRICS_COUNT = 30
 
 
 class PeelDesc(ctypes.Structure):

@@ -32,7 +32,10 @@ METRIC_NAMES = (
     "matched_patch_edges",
     "edge_hash_misses",
 )
-EXTRA_METRICS = ("filtered_predicates", "fp64_fallbacks", "frontier_slots", "max_edge_vertices", "unique_boundary_vertices")
+EXTRA_METRICS = ("filtered_containment", "fp64_containment", "frontier_slots", "max_edge_vertices",
+                 "unique_boundary_vertices", "filtered_straddle", "fp64_straddle", "filtered_front_segment",
+                 "fp64_front_segment", "filtered_depth_order", "fp64_depth_order")
+FILTER_CLASSES = ("containment", "straddle", "front_segment", "depth_order")
 
 
 @dataclass(frozen=True)
@@ -60,10 +63,14 @@ class EventPeeler(_Workspace):
     """warp_peel.EventPeeler interface. Requires resolved surface intersections.
 
     ``filter_fp32`` (an extension; the reference has no such switch) decides
-    the rays' point-in-polygon orientation tests in binary32 under a static
-    forward error bound and evaluates only uncertain tests in binary64; the
-    outputs are bit-identical to the binary64 route, and ``read_status``
-    reports ``filtered_predicates`` and ``fp64_fallbacks``."""
+    four predicate classes in binary32 under static forward error bounds and
+    evaluates only uncertain tests in binary64: the rays' point-in-polygon
+    orientations (``containment``), the subdivision's straddle signs
+    (``straddle``) and ring-edge tests of ``front_segment``, and the depth
+    comparisons that place a patch in its ray's hit order (``depth_order``).
+    The outputs are bit-identical to the binary64 route; ``read_status``
+    reports ``filtered_<class>`` and ``fp64_<class>`` per class and their sums
+    ``filtered_predicates`` and ``fp64_fallbacks``."""
 
     _route = _lib.ROUTE_EVENTS
 
@@ -121,5 +128,7 @@ class EventPeeler(_Workspace):
         for name in ("peeling_passes", "max_hidden_pieces"):
             info.pop(name)
         info.update(zip(METRIC_NAMES, map(int, metrics)))
-        info.update(zip(EXTRA_METRICS, map(int, metrics[19:24])))
+        info.update(zip(EXTRA_METRICS, map(int, metrics[19:30])))
+        info["filtered_predicates"] = sum(info[f"filtered_{c}"] for c in FILTER_CLASSES)
+        info["fp64_fallbacks"] = sum(info[f"fp64_{c}"] for c in FILTER_CLASSES)
         return info

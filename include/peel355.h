@@ -224,7 +224,7 @@ int pool_mlp_adam(float *params, const float *grad, float *m, float *v, int32_t 
  * Calls on one GPU share its array: calls that overlap on one GPU (other
  * streams, other threads) race and must be ordered by the caller. */
 #define PEEL_STATIC_SCRATCH_BYTES (256ull << 20)
-#define PEEL_METRICS_COUNT 24
+#define PEEL_METRICS_COUNT 30
 typedef struct PeelOutputs {
   double *output_xy;
   int32_t *output_n;

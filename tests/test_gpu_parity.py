@@ -118,8 +118,10 @@ def test_stack_65536_closed_form():
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", CASES)
 def test_fixture_fp32_filter_is_bit_identical(name):
-    """binary32-filtered ray containment gives the binary64 route's output bit
-    for bit (checked against the CPU restatement) and counts its decisions."""
+    """The binary32 filters (ray containment, subdivision straddle signs,
+    front_segment edge tests, rank depth order) give the binary64 route's
+    output bit for bit (checked against the CPU restatement) and count their
+    decisions."""
     print(f"[parity] {name} fp32 filter", flush=True)
     p, info = compare(oracle.load_scene(fixture_path(name))[:4], filter_fp32=True)
     if info["ray_candidate_tests"]:
@@ -135,6 +137,15 @@ def test_torus_knot_fp32_filter_bit_parity(grid):
     p, info = compare(scenes.torus_knot(segments=grid[0], sides=grid[1]), filter_fp32=True)
     total = info["filtered_predicates"] + info["fp64_fallbacks"]
     assert total > 0 and info["fp64_fallbacks"] < total
+    # every filtered class decides tests in binary32; the sign classes decide
+    # most of theirs (a front_segment edge that a segment crosses needs its
+    # binary64 orientations for the clipped interval whatever the filter says)
+    from hip_peel.events import FILTER_CLASSES
+
+    for c in FILTER_CLASSES:
+        f, b = info[f"filtered_{c}"], info[f"fp64_{c}"]
+        print(f"[parity] {c}: {f} binary32, {b} binary64", flush=True)
+        assert f > 0 and (c == "front_segment" or f > b), c
 
 
 @pytest.mark.gpu
