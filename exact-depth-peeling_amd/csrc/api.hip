@@ -145,10 +145,7 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
   c.take(w.inverse, 16);
   c.take(w.canonical, nv1);
   c.take(w.vhash, w.vhash_size);
-  c.take(w.ekeys, w.ehash_size);
-  c.take(w.ecount, w.ehash_size);
-  c.take(w.efirst, w.ehash_size);
-  c.take(w.esecond, w.ehash_size);
+  c.take(w.etable, w.ehash_size);
   c.take(w.masks, 3 * nf1);
   c.take(w.src_xy, SRCV * nf1);
   c.take(w.src_n, nf1);

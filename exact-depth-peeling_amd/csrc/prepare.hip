@@ -23,10 +23,7 @@ __global__ void k_reset(Ws w) {
   if (i < M_COUNT) w.metrics[i] = 0;
   if (i < C_COUNT) w.control[i] = 0;
   for (int j = i; j < w.vhash_size; j += gridDim.x * blockDim.x) w.vhash[j] = -1;
-  for (int j = i; j < w.ehash_size; j += gridDim.x * blockDim.x) {
-    w.ekeys[j] = ~0ull;
-    w.ecount[j] = 0;
-  }
+  for (int j = i; j < w.ehash_size; j += gridDim.x * blockDim.x) w.etable[j] = EdgeSlot{~0ull, 0ull};
   for (int j = i; j < w.shash_size; j += gridDim.x * blockDim.x) w.shash[j] = -1;
   for (int j = i; j < 2 * w.nseg; j += gridDim.x * blockDim.x) w.seg_near[j] = ~0ull;
 }
@@ -418,26 +415,33 @@ __global__ void k_project(Ws w) {
 // ---------------------------------------------------------------------------
 // Undirected edge incidences (warp_peel/kernels.py:471-520): an edge is
 // internal iff exactly two incidences with different start vertices and equal
-// facing; everything else is a contour.
-__device__ __forceinline__ unsigned long long edge_key(const Ws &w, int e) {
+// facing; everything else is a contour. Each incidence adds its direction and
+// facing to its edge's slot, so deciding an incidence reads its face, its two
+// canonical vertices and one slot (key and summary share a 16-byte record)
+// instead of the other incidence's face, vertices and facing.
+__device__ __forceinline__ unsigned long long edge_key(const Ws &w, int e, bool &from_min) {
   int f = e / 3, j = e % 3;
   int a = w.canonical[w.faces[3 * f + j]];
   int b = w.canonical[w.faces[3 * f + (j + 1) % 3]];
+  from_min = a <= b;
   return ((unsigned long long)(uint32_t)min(a, b) << 32) | (uint32_t)max(a, b);
 }
 
 __global__ void k_edge_insert(Ws w) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * w.nf || w.status[ST_ERR] != 0) return;
-  unsigned long long key = edge_key(w, e);
+  bool from_min;
+  unsigned long long key = edge_key(w, e, from_min);
+  int face = w.facing[e / 3];
+  unsigned long long add = 1ull | ((unsigned long long)from_min << EDGE_DIR_SHIFT) |
+                           ((unsigned long long)(face > 0) << EDGE_FRONT_SHIFT) |
+                           ((unsigned long long)(face < 0) << EDGE_BACK_SHIFT);
   int mask = w.ehash_size - 1;
   int slot = (int)(mix64(key) & (uint64_t)mask);
   for (int probe = 0; probe < w.ehash_size; ++probe) {
-    unsigned long long cur = atomicCAS(&w.ekeys[slot], ~0ull, key);
+    unsigned long long cur = atomicCAS(&w.etable[slot].key, ~0ull, key);
     if (cur == ~0ull || cur == key) {
-      int c = atomicAdd(&w.ecount[slot], 1);
-      if (c == 0) w.efirst[slot] = e;
-      if (c == 1) w.esecond[slot] = e;
+      atomicAdd(&w.etable[slot].info, add);
       return;
     }
     slot = (slot + 1) & mask;
@@ -451,22 +455,26 @@ __global__ void k_edge_mark(Ws w) {
     w.masks[e] = 0;
     return;
   }
-  unsigned long long key = edge_key(w, e);
+  bool from_min;
+  unsigned long long key = edge_key(w, e, from_min);
   int mask = w.ehash_size - 1;
   int slot = (int)(mix64(key) & (uint64_t)mask);
   int contour = 1;
   for (int probe = 0; probe < w.ehash_size; ++probe) {
-    unsigned long long cur = w.ekeys[slot];
-    if (cur == key) {
-      if (w.ecount[slot] == 2) {
-        int e0 = w.efirst[slot], e1 = w.esecond[slot];
-        int a = w.canonical[w.faces[3 * (e0 / 3) + e0 % 3]];
-        int b = w.canonical[w.faces[3 * (e1 / 3) + e1 % 3]];
-        if (a != b && w.facing[e0 / 3] == w.facing[e1 / 3]) contour = 0;
+    EdgeSlot cur = w.etable[slot];
+    if (cur.key == key) {
+      unsigned long long info = cur.info;
+      if ((uint32_t)info == 2u) {
+        // two incidences: opposite directions (one starts at the smaller
+        // vertex), and both front-, both back- or both zero-facing
+        int dir = (int)((info >> EDGE_DIR_SHIFT) & 0xffff);
+        int front = (int)((info >> EDGE_FRONT_SHIFT) & 0xff), back = (int)(info >> EDGE_BACK_SHIFT);
+        bool same_facing = front == 2 || back == 2 || (front == 0 && back == 0);
+        if (dir == 1 && same_facing) contour = 0;
       }
       break;
     }
-    if (cur == ~0ull) break;
+    if (cur.key == ~0ull) break;
     slot = (slot + 1) & mask;
   }
   w.masks[e] = contour;

@@ -565,7 +565,20 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
       store_hits(w, hit, leaf, i, x);
       count += hit;
     }
-    if (count) atomicAdd(&w.group_counts[i], count);
+    // Consecutive tasks are parts of one job, so the lanes of a wave usually
+    // add to one ray's count: those that share the first contributing lane's
+    // ray add through it with one atomic (the 65,536-layer stack's single ray
+    // took 65,536 atomics on one address), the others add their own.
+    const int lane = threadIdx.x & 63;
+    uint64_t adding = __ballot(count != 0);
+    if (adding) {
+      int leader = __ffsll((unsigned long long)adding) - 1;
+      int ray = __shfl(i, leader);
+      bool same = count != 0 && i == ray;
+      long long shared = wave_sum(same ? count : 0);
+      if (lane == leader) atomicAdd(&w.group_counts[ray], (int)shared);
+      if (count != 0 && !same) atomicAdd(&w.group_counts[i], count);
+    }
   }
   wave_add_metric(&w.metrics[M_RAY_TESTS], visits);
   if (w.filter32) {

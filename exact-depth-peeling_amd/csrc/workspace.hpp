@@ -17,6 +17,16 @@ struct Caps {
   int edge_vertices;  // edge-noding candidates (<= 128)
 };
 
+// One slot of the undirected-edge table: the key (the two canonical vertex
+// ids) and, in one word that every incidence adds to, the incidence count
+// (bits 0-31), the incidences that start at the smaller vertex (bits 32-47),
+// and those of front-facing (48-55) and back-facing (56-63) faces. The fields
+// above the count are read only when the count is 2, where none can carry.
+struct EdgeSlot {
+  unsigned long long key, info;
+};
+constexpr int EDGE_DIR_SHIFT = 32, EDGE_FRONT_SHIFT = 48, EDGE_BACK_SHIFT = 56;
+
 struct Ws {
   // inputs (caller-owned device arrays)
   const double *vertices;  // [nv][3]
@@ -38,8 +48,7 @@ struct Ws {
   int *canonical;          // [nv]
   int *vhash;              // [vhash_size] vertex ids
   int vhash_size;
-  unsigned long long *ekeys;  // [ehash_size]
-  int *ecount, *efirst, *esecond;  // [ehash_size]
+  EdgeSlot *etable;        // [ehash_size] undirected edge -> incidence summary
   int ehash_size;
   int *masks;              // [3 nf] 1 = contour edge
 
