@@ -38,34 +38,10 @@ __device__ __forceinline__ int pedges(const Ws &w) {
 }
 __device__ __forceinline__ int eslots_live(const Ws &w) { return live_table(pedges(w), w.ehash2_size); }
 
-__global__ void k_connect_reset(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int P = npolys_c(w);
-  for (int j = i; j < P; j += gridDim.x * blockDim.x) {
-    w.parents[j] = j;
-    w.reps[j] = j;
-    w.choices[j] = 0x7fffffff;
-  }
-  if (i == 0) {
-    // components.py:40-52: connect only after a split or with an internal edge
-    int any = (w.metrics[M_SPLITS] > 0) || (w.control[C_INTERNAL] != 0);
-    w.control[C_CONNECT] = any;
-    if (!any && w.propagate_ranks) w.metrics[M_COMPONENTS] = npolys_c(w);
-  }
-}
-
-__global__ void k_patch_counts(Ws w) {
-  bool ok = w.status[ST_ERR] == 0;
-  ITEM_LOOP(i, npolys_c(w)) w.vcounts[i] = ok ? w.out_n[i] : 0;
-}
-
-__global__ void k_patch_edges(Ws w) {
-  int total = w.control[C_PEDGES];
-  if (total > w.cap.vertices) set_error(w.status, ERR_VERTS);
-}
-
-// Clear the live part of the reversed-edge table and the partner slots.
+// Check the patch-edge total of the ring-offset scan, then clear the live part
+// of the reversed-edge table and the partner slots.
 __global__ void k_eslots_reset(Ws w) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && w.control[C_PEDGES] > w.cap.vertices) set_error(w.status, ERR_VERTS);
   if (!connecting(w)) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int size = eslots_live(w), n = pedges(w);
@@ -266,13 +242,10 @@ static inline dim3 grid_for(long n, int block = 256) {
   return dim3((unsigned)(g < 1 ? 1 : g));
 }
 
+// After the rank stage's reset kernel, which also resets the connectivity.
 void enqueue_connect(const Ws &w, hipStream_t s) {
-  long g = ((long)w.cap.regions + 255) / 256;
-  hipLaunchKernelGGL(k_connect_reset, dim3((unsigned)(g > 1024 ? 1024 : g)), dim3(256), 0, s, w);
-  hipLaunchKernelGGL(k_patch_counts, stride_grid(w.cap.regions), dim3(256), 0, s, w);
-  scan_exclusive(s, w.vcounts, w.voffsets, w.status + ST_POLYS, 0, w.cap.regions,
-                 w.control + C_PEDGES, w.scan);
-  hipLaunchKernelGGL(k_patch_edges, dim3(1), dim3(1), 0, s, w);
+  // ring offsets of the patches: an exclusive scan of their vertex counts
+  scan_exclusive(s, w.out_n, w.voffsets, w.status + ST_POLYS, 0, w.cap.regions, w.control + C_PEDGES, w.scan);
   long eg = ((long)std::max(w.ehash2_size, w.cap.vertices) + 255) / 256;
   hipLaunchKernelGGL(k_eslots_reset, dim3((unsigned)(eg > 1024 ? 1024 : eg)), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_edge_owners, stride_grid(w.cap.regions), dim3(256), 0, s, w);

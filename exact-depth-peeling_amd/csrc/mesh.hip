@@ -22,15 +22,20 @@ __device__ __forceinline__ int npolys_m(const Ws &w) {
   return p < w.cap.regions ? p : w.cap.regions;
 }
 
-__global__ void k_vertex_counts(Ws w) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) w.control[C_CROWDED] = 0;
-  bool ok = w.status[ST_ERR] == 0;
-  ITEM_LOOP(i, npolys_m(w)) w.vcounts[i] = ok ? w.out_n[i] : 0;
-}
-
 __device__ void scatter_one(const Ws &w, int i);
 
+// After the exclusive scan of the polygons' vertex counts (their ring
+// offsets): check and publish the boundary-vertex total, clear the live part
+// of the point table for k_point_dedupe, and scatter every ring's vertices.
 __global__ void k_scatter(Ws w) {
+  int total = load_i(w.control + C_NBV);
+  int live = total > w.cap.vertices ? 0 : total;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (total > w.cap.vertices) set_error(w.status, ERR_VERTS);
+    w.status[ST_BVERTS] = live;
+  }
+  int size = live_table(live, w.phash_size);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < size; j += gridDim.x * blockDim.x) w.phash[j] = -1;
   if (w.status[ST_ERR] != 0) return;
   ITEM_LOOP(i, npolys_m(w)) scatter_one(w, i);
 }
@@ -69,15 +74,18 @@ __device__ __forceinline__ int phash_live(const Ws &w) {
   return live_table(load_i(w.status + ST_BVERTS), w.phash_size);
 }
 
-__global__ void k_phash_reset(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int size = phash_live(w);
-  for (int j = i; j < size; j += gridDim.x * blockDim.x) w.phash[j] = -1;
-}
-
 __device__ void dedupe_one(const Ws &w, int i);
+__device__ int grid_side(const Ws &w);
+__device__ __forceinline__ long cells_live(const Ws &w, int G);
 
+// Deduplicate the boundary vertices per layer; beside it, plan the per-layer
+// cell grid from the vertex total and clear its live cells for k_cell_insert.
 __global__ void k_point_dedupe(Ws w) {
+  int G = grid_side(w);
+  if (blockIdx.x == 0 && threadIdx.x == 0) w.control[C_GRID] = G;
+  long cells = cells_live(w, G);
+  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < cells; j += (long)gridDim.x * blockDim.x)
+    w.cell_heads[j] = -1;
   if (w.status[ST_ERR] != 0) return;
   ITEM_LOOP(i, load_i(w.status + ST_BVERTS)) dedupe_one(w, i);
 }
@@ -134,23 +142,19 @@ __device__ __forceinline__ int cell_index(double v, int G) {
 
 // G: power of two with G^2 about the distinct vertices per layer (copies are
 // about twice the distinct vertices), between 8 and 4096.
-__global__ void k_grid_plan(Ws w) {
+// Cells per axis of each layer's grid: about two boundary vertices per cell,
+// within the cell pool.
+__device__ int grid_side(const Ws &w) {
   int layers = max(1, w.status[ST_LAYERS]);
   double per_layer = (double)load_i(w.status + ST_BVERTS) / (2.0 * (double)layers);
   int G = 8;
   while (G < 4096 && (double)G * (double)G < per_layer) G *= 2;
   while (G > 1 && (long)layers * G * G > w.cell_capacity) G /= 2;
-  w.control[C_GRID] = G;
+  return G;
 }
 
 __device__ __forceinline__ long cells_live(const Ws &w, int G) {
   return (long)max(1, w.status[ST_LAYERS]) * G * G;
-}
-
-__global__ void k_cells_reset(Ws w) {
-  long size = cells_live(w, load_i(w.control + C_GRID));
-  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < size; j += (long)gridDim.x * blockDim.x)
-    w.cell_heads[j] = -1;
 }
 
 __device__ void cell_insert_one(const Ws &w, int i, int G);
@@ -544,29 +548,12 @@ static inline dim3 grid_for(long n, int block = 256) {
   return dim3((unsigned)(g < 1 ? 1 : g));
 }
 
-__global__ void k_check_vertices(Ws w) {
-  int total = w.control[C_NBV];
-  if (total > w.cap.vertices) set_error(w.status, ERR_VERTS);
-  w.status[ST_BVERTS] = total > w.cap.vertices ? 0 : total;
-}
-
 void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t) {
-  hipLaunchKernelGGL(k_vertex_counts, stride_grid(w.cap.regions), dim3(256), 0, s, w);
-  scan_exclusive(s, w.vcounts, w.voffsets, w.status + ST_POLYS, 0, w.cap.regions, w.control + C_NBV,
-                 w.scan);
-  hipLaunchKernelGGL(k_check_vertices, dim3(1), dim3(1), 0, s, w);
+  // ring offsets: an exclusive scan of the polygons' vertex counts
+  scan_exclusive(s, w.out_n, w.voffsets, w.status + ST_POLYS, 0, w.cap.regions, w.control + C_NBV, w.scan);
   hipLaunchKernelGGL(k_scatter, stride_grid(w.cap.regions), dim3(256), 0, s, w);
-  {
-    long g = ((long)w.phash_size + 255) / 256;
-    hipLaunchKernelGGL(k_phash_reset, dim3((unsigned)(g > 1024 ? 1024 : g)), dim3(256), 0, s, w);
-  }
   hipLaunchKernelGGL(k_point_dedupe, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "scatter_vertices");
-  hipLaunchKernelGGL(k_grid_plan, dim3(1), dim3(1), 0, s, w);
-  {
-    long g = (w.cell_capacity + 255) / 256;
-    hipLaunchKernelGGL(k_cells_reset, dim3((unsigned)(g > 1024 ? 1024 : g)), dim3(256), 0, s, w);
-  }
   hipLaunchKernelGGL(k_cell_insert, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "vertex_grid");
   hipLaunchKernelGGL(k_mesh_edges, stride_grid(w.cap.vertices), dim3(256), 0, s, w);

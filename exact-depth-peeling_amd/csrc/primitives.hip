@@ -346,17 +346,17 @@ void scan_exclusive(hipStream_t s, const int *in, int *out, const int *count, in
 
 // ---------------------------------------------------------------------------
 // Morton hierarchy (warp_peel/kernels.py:67-116, solver.py:34-89).
-__global__ void h_size(const int *count, int count_static, int capacity, int *leaves) {
-  int n = live_count(count, count_static, capacity);
-  int s = 1;
-  while (s < n) s <<= 1;
-  *leaves = s;
-}
-
+// Morton keys of the live boxes; thread 0 also sizes the tree: the power of two
+// at least the live count.
 __global__ void h_keys(const Box *boxes, const int *count, int count_static, int capacity,
-                       int layered, uint64_t *keys, int *ids) {
+                       int layered, uint64_t *keys, int *ids, int *leaves) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int n = live_count(count, count_static, capacity);
+  if (i == 0) {
+    int s = 1;
+    while (s < n) s <<= 1;
+    *leaves = s;
+  }
   if (i >= n) return;
   ids[i] = i;
   uint64_t key = ~0ull;
@@ -428,10 +428,9 @@ __global__ __launch_bounds__(256) void h_levels(Box *nodes, const int *leaves, i
 void build_hierarchy(hipStream_t s, const Box *boxes, const int *count, int count_static,
                      int capacity, bool layered, Hierarchy &h, const SortScratch &scratch) {
   int cap = capacity < h.size ? capacity : h.size;
-  hipLaunchKernelGGL(h_size, dim3(1), dim3(1), 0, s, count, count_static, cap, h.leaves);
   int grid = (cap + 255) / 256;
   hipLaunchKernelGGL(h_keys, dim3(grid < 1 ? 1 : grid), dim3(256), 0, s, boxes, count, count_static,
-                     cap, (int)layered, h.keys, h.ids);
+                     cap, (int)layered, h.keys, h.ids, h.leaves);
   radix_sort(s, h.keys, h.ids, h.alt_keys, h.alt_ids, count, count_static, cap, 0, layered ? 64 : 32,
              scratch);
   hipLaunchKernelGGL(h_leaves, dim3((h.size + 255) / 256), dim3(256), 0, s, boxes, count,

@@ -16,6 +16,8 @@
 // the patch that claimed the sample's hash slot first (a hash table instead of
 // the reference's two float64 radix sorts and a binary search). Which member
 // leads does not matter: all members cast the same ray.
+#include <algorithm>
+
 #include "workspace.hpp"
 
 namespace peel {
@@ -31,6 +33,24 @@ __device__ __forceinline__ uint64_t point_hash(double2 x) {
 
 __device__ __forceinline__ int qhash_live(const Ws &w) { return live_table(npolys(w), w.qhash_size); }
 
+// Connectivity reset (components.py:32-52): every patch its own parent and
+// representative, and whether patches connect at all: only after a split or
+// with an internal mesh edge.
+__device__ __forceinline__ void connect_reset(const Ws &w, int i) {
+  int P = npolys(w);
+  for (int j = i; j < P; j += gridDim.x * blockDim.x) {
+    w.parents[j] = j;
+    w.reps[j] = j;
+    w.choices[j] = 0x7fffffff;
+  }
+  if (i == 0) {
+    int any = (w.metrics[M_SPLITS] > 0) || (w.control[C_INTERNAL] != 0);
+    w.control[C_CONNECT] = any;
+    if (!any && w.propagate_ranks) w.metrics[M_COMPONENTS] = P;
+  }
+}
+
+// Resets of the rank stage and of the connectivity it starts with, one launch.
 __global__ void k_rank_reset(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int size = qhash_live(w);
@@ -50,7 +70,11 @@ __global__ void k_rank_reset(Ws w) {
     w.control[C_HUGE] = 0;
     w.control[C_HUGE_SYNC] = 0;
     w.control[C_LEADERS] = 0;
+    // subdivision totals, for read_status
+    w.metrics[M_SLOTS] = w.control[C_ALLOC];
+    if (w.nf == 0) w.metrics[M_ROUNDS] = 0;
   }
+  if (w.propagate_ranks || w.merge_regions) connect_reset(w, i);
 }
 
 // Canonical sample of the representative patch: centroid summed from the
@@ -587,12 +611,6 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   }
 }
 
-__global__ void k_check_hits(Ws w) {
-  long long h = w.metrics[M_RAY_HITS];
-  if (h > w.cap.hits) set_error(w.status, ERR_HITS);
-  w.control[C_NHITS] = (int)(h < w.cap.hits ? h : w.cap.hits);
-}
-
 // ---------------------------------------------------------------------------
 // Hit order (group, depth, source). Each group's hits form one contiguous
 // segment at its scanned offset; segments are sorted independently: up to 64
@@ -612,8 +630,12 @@ __device__ __forceinline__ bool hit_less(uint64_t ka, int sa, uint64_t kb, int s
 // atomics on one counter); the other lanes, usually of small groups, take
 // theirs one by one.
 __global__ void k_hit_scatter(Ws w) {
+  // the ray stages' hit total against the pool (check_hits,
+  // event_kernels.py:300-309); only stored hits are scattered
+  const long long total = w.metrics[M_RAY_HITS];
+  if (blockIdx.x == 0 && threadIdx.x == 0 && total > w.cap.hits) set_error(w.status, ERR_HITS);
   if (w.status[ST_ERR] != 0) return;
-  const int n = load_i(w.control + C_NHITS);
+  const int n = (int)(total < w.cap.hits ? total : w.cap.hits);
   const int lane = threadIdx.x & 63;
   const uint64_t below = (1ull << lane) - 1;
   WAVE_LOOP(base, n) {
@@ -1033,7 +1055,7 @@ static inline dim3 grid_for(long n, int block = 256) {
 }
 
 void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
-  long qgrid = ((long)w.qhash_size + 255) / 256;
+  long qgrid = ((long)std::max(w.qhash_size, w.cap.regions) + 255) / 256;
   if (qgrid > 1024) qgrid = 1024;
   hipLaunchKernelGGL(k_rank_reset, dim3((unsigned)qgrid), dim3(256), 0, s, w);
   if (w.propagate_ranks || w.merge_regions) enqueue_connect(w, s);
@@ -1060,7 +1082,6 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
     hipLaunchKernelGGL(k_heavy_hits, hg, dim3(256), 0, s, w, (int)hg.x * 256);
   }
   mark(t, s, "k_heavy_hits");
-  hipLaunchKernelGGL(k_check_hits, dim3(1), dim3(1), 0, s, w);
   scan_exclusive(s, w.group_counts, w.group_offsets, w.status + ST_POLYS, 0, w.cap.regions,
                  nullptr, w.scan);
   hipLaunchKernelGGL(k_hit_scatter, stride_grid(w.cap.hits), dim3(256), 0, s, w);

@@ -28,15 +28,8 @@
 
 namespace peel {
 
-__global__ void k_frontier_reset(Ws w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int j = i; j < w.cap.frontier; j += gridDim.x * blockDim.x) w.fr_ready[j] = 0;
-  if (i == 0) {
-    w.control[C_ALLOC] = 0;
-    w.control[C_HEAD] = 0;
-    w.control[C_PENDING] = w.nf;
-  }
-}
+// The queue's ready flags and counters (C_ALLOC, C_HEAD, C_PENDING = nf) are
+// set by the pipeline's first kernel, k_reset.
 
 struct Slot {
   const double2 *xy;  // vertex j at xy[j * stride]
@@ -370,21 +363,9 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
     atomicMax((unsigned long long *)&w.metrics[M_ROUNDS], (unsigned long long)(max_gen + 1));
 }
 
-__global__ void k_subdivide_finish(Ws w) {
-  // frontier slots used and the live-count peak, for read_status
-  w.metrics[M_SLOTS] = w.control[C_ALLOC];
-  if (w.nf == 0) w.metrics[M_ROUNDS] = 0;
-}
-
 void enqueue_subdivide(const Ws &w, hipStream_t s, Timeline *t) {
-  long grid = ((long)w.cap.frontier + 255) / 256;
-  if (grid > 1024) grid = 1024;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(k_frontier_reset, dim3((unsigned)grid), dim3(256), 0, s, w);
-  mark(t, s, "frontier_reset");
   hipLaunchKernelGGL(k_subdivide, dim3(w.persistent_blocks), dim3(SUB_THREADS), 0, s, w);
   mark(t, s, "k_subdivide");
-  hipLaunchKernelGGL(k_subdivide_finish, dim3(1), dim3(1), 0, s, w);
 }
 
 }  // namespace peel
