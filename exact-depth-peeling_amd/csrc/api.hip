@@ -121,6 +121,7 @@ static void carve_hierarchy(Carver &c, Hierarchy &h) {
   c.take(h.alt_ids, h.size);
   c.take(h.nodes, 2L * h.size);
   c.take(h.leaves, 1);
+  c.take(h.level_counters, HIER_COUNTERS);
 }
 
 // Every pool of a planned workspace. `outputs` false leaves the caller-owned
@@ -249,8 +250,10 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
   c.take(w.sort.tickets, RS_MAX_SWEEPS);
   c.take(w.sort.tiles, (long)RS_MAX_SWEEPS * w.sort.max_blocks * 256);
   c.take(w.sort.flags, RS_MAX_SWEEPS);
-  c.take(w.scan.block_sums, w.scan.max_blocks);
+  c.take(w.scan.words, w.scan.max_blocks);
+  c.take(w.scan.counters, 2);
   w.sort.status = w.status;
+  w.scan.status = w.status;
 }
 
 extern "C" {

@@ -277,44 +277,34 @@ void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan: tile sums, then tile scans offset by the sums of the tiles before them.
+// Exclusive scan in one launch with decoupled look-back, as the radix passes:
+// only blocks with a live tile take a ticket (in execution order), each tile
+// publishes its total, the first wave walks back over up to 64 predecessors'
+// words at a time to the nearest published prefix, and the tile publishes its
+// own prefix before it writes. The last tile to finish clears the words and
+// counters for the next scan. Words pack a 2-bit state (0 none, 1 tile total,
+// 2 inclusive prefix) over a 62-bit value.
 constexpr int SC_THREADS = 256, SC_ITEMS = 16, SC_TILE = SC_THREADS * SC_ITEMS;
+constexpr unsigned long long SC_TOTAL = 1ull << 62, SC_PREFIX = 2ull << 62, SC_VALUE = (1ull << 62) - 1;
 
-
-__global__ __launch_bounds__(SC_THREADS) void sc_reduce(const int *in, const int *count,
-                                                         int count_static, int capacity,
-                                                         int *block_sums) {
+__global__ __launch_bounds__(SC_THREADS) void sc_onepass(const int *in, int *out, const int *count,
+                                                          int count_static, int capacity, int *total_out,
+                                                          unsigned long long *words, int *counters,
+                                                          int *status) {
   __shared__ int lds[4];
-  int n = live_count(count, count_static, capacity);
-  int begin = blockIdx.x * SC_TILE;
-  if (begin >= n) return;
-  int local = 0;
-  for (int i = begin + threadIdx.x; i < min(n, begin + SC_TILE); i += SC_THREADS) local += in[i];
-  int total;
-  block_exclusive(local, lds, total);
-  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
-}
-
-// Each tile adds up the sums of the tiles before it (at most a few hundred,
-// two per thread and one block reduction), so the scan takes two launches;
-// the last tile writes the total.
-__global__ __launch_bounds__(SC_THREADS) void sc_apply(const int *in, int *out, const int *count,
-                                                        int count_static, int capacity,
-                                                        const int *block_sums, int *total_out) {
-  __shared__ int lds[4];
+  __shared__ int tile_shared, last_shared;
+  __shared__ long long before_shared;
   int n = live_count(count, count_static, capacity);
   if (n == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && total_out) *total_out = 0;
     return;
   }
-  int begin = blockIdx.x * SC_TILE;
-  if (begin >= n) return;
-  int part = 0;
-  for (int b = threadIdx.x; b < (int)blockIdx.x; b += SC_THREADS) part += block_sums[b];
-  int offset;
-  block_exclusive(part, lds, offset);
-  // each thread scans SC_ITEMS consecutive elements
-  int first = begin + threadIdx.x * SC_ITEMS;
+  const int tiles = (n + SC_TILE - 1) / SC_TILE;
+  if ((int)blockIdx.x >= tiles) return;
+  if (threadIdx.x == 0) tile_shared = atomicAdd(counters, 1);
+  __syncthreads();
+  const int tile = tile_shared;
+  const int first = tile * SC_TILE + threadIdx.x * SC_ITEMS;
   int vals[SC_ITEMS];
   int local = 0;
 #pragma unroll
@@ -324,13 +314,63 @@ __global__ __launch_bounds__(SC_THREADS) void sc_apply(const int *in, int *out, 
     local += vals[j];
   }
   int total;
-  int running = block_exclusive(local, lds, total) + offset;
-  if (threadIdx.x == 0 && total_out && begin + SC_TILE >= n) *total_out = offset + total;
+  int running = block_exclusive(local, lds, total);
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    long long before = 0;
+    if (tile == 0) {
+      if (lane == 0)
+        __hip_atomic_store(words, SC_PREFIX | (unsigned long long)total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      if (lane == 0)
+        __hip_atomic_store(words + tile, SC_TOTAL | (unsigned long long)total, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      int t = tile - 1, spins = 0;  // nearest predecessor not yet added
+      while (true) {
+        int j = t - lane;
+        unsigned long long word =
+            j >= 0 ? __hip_atomic_load(words + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : SC_PREFIX;
+        uint64_t pending = __ballot((word >> 62) == 0);
+        uint64_t prefix = __ballot((word >> 62) == 2);
+        int stop = prefix ? __ffsll((unsigned long long)prefix) - 1 : 63;      // last lane this window needs
+        int gap = pending ? __ffsll((unsigned long long)pending) - 1 : 64;     // first unpublished lane
+        int take = gap <= stop ? gap : stop + 1;                                // lanes [0, take) are added
+        before += wave_sum(lane < take ? (long long)(word & SC_VALUE) : 0ll);
+        if (gap > stop && prefix) break;
+        t -= take;
+        if (gap <= stop) {
+          if (++spins > LB_SPIN_LIMIT) {
+            if (lane == 0) set_error(status, ERR_WAIT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      if (lane == 0)
+        __hip_atomic_store(words + tile, SC_PREFIX | (unsigned long long)(before + total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) before_shared = before;
+  }
+  __syncthreads();
+  const int offset = (int)before_shared;
+  if (threadIdx.x == 0 && total_out && tile == tiles - 1) *total_out = offset + total;
+  running += offset;
 #pragma unroll
   for (int j = 0; j < SC_ITEMS; ++j) {
     int i = first + j;
     if (i < n) out[i] = running;
     running += vals[j];
+  }
+  // the last tile to finish leaves the words and counters zero
+  if (threadIdx.x == 0) last_shared = atomicAdd(counters + 1, 1) == tiles - 1;
+  __syncthreads();
+  if (last_shared) {
+    for (int j = threadIdx.x; j < tiles; j += SC_THREADS) words[j] = 0ull;
+    if (threadIdx.x == 0) {
+      counters[0] = 0;
+      counters[1] = 0;
+    }
   }
 }
 
@@ -338,10 +378,8 @@ void scan_exclusive(hipStream_t s, const int *in, int *out, const int *count, in
                     int capacity, int *total, const ScanScratch &scratch) {
   int blocks = (capacity + SC_TILE - 1) / SC_TILE;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(sc_reduce, dim3(blocks), dim3(SC_THREADS), 0, s, in, count, count_static,
-                     capacity, scratch.block_sums);
-  hipLaunchKernelGGL(sc_apply, dim3(blocks), dim3(SC_THREADS), 0, s, in, out, count,
-                     count_static, capacity, scratch.block_sums, total);
+  hipLaunchKernelGGL(sc_onepass, dim3(blocks), dim3(SC_THREADS), 0, s, in, out, count, count_static, capacity,
+                     total, scratch.words, scratch.counters, scratch.status);
 }
 
 // ---------------------------------------------------------------------------
@@ -396,32 +434,53 @@ __device__ __forceinline__ Box unite(const Box &a, const Box &b) {
   return r;
 }
 
-// Launch `stage` builds levels [9 stage, 9 stage + 9) above the leaves: block b
-// reduces up to 512 nodes of that bottom level inside LDS.
-__global__ __launch_bounds__(256) void h_levels(Box *nodes, const int *leaves, int stage) {
-  __shared__ Box lds[512];
-  int bottom = *leaves >> (9 * stage);
-  if (bottom <= 1) return;
-  int levels = 0;
-  while ((1 << (levels + 1)) <= bottom && levels < 9) ++levels;
-  int width = 1 << levels;
-  int first = bottom + blockIdx.x * width;
-  if (blockIdx.x * width >= bottom) return;
-  for (int i = threadIdx.x; i < width; i += 256) lds[i] = nodes[first + i];
-  __syncthreads();
-  int count = width, start = first;
-  for (int l = 0; l < levels; ++l) {
-    int half = count >> 1;
-    Box parent;
-    if ((int)threadIdx.x < half) parent = unite(lds[2 * threadIdx.x], lds[2 * threadIdx.x + 1]);
+// Every internal level in one launch. Each block reduces a group of up to 512
+// nodes of the bottom level in LDS, nine levels at a time; the block that
+// completes a group of 512 such results (an arrival counter, release and
+// acquire fences around it) goes on to reduce that group, and so on to the
+// root. Each stage's counters are cleared by the block that goes on.
+__global__ __launch_bounds__(256) void h_levels(Box *nodes, const int *leaves, int *counters) {
+  __shared__ Box lds[HIER_GROUP];
+  __shared__ int proceed;
+  int bottom = *leaves;
+  int group = blockIdx.x;
+  int width = bottom < HIER_GROUP ? bottom : HIER_GROUP;
+  if (bottom <= 1 || group * width >= bottom) return;
+  for (int stage = 0;; ++stage) {
+    int levels = 0;
+    while ((1 << (levels + 1)) <= width) ++levels;
+    int first = bottom + group * width;
+    for (int i = threadIdx.x; i < width; i += 256) lds[i] = nodes[first + i];
     __syncthreads();
-    start >>= 1;
-    if ((int)threadIdx.x < half) {
-      lds[threadIdx.x] = parent;
-      nodes[start + threadIdx.x] = parent;
+    int count = width, start = first;
+    for (int l = 0; l < levels; ++l) {
+      int half = count >> 1;
+      Box parent;
+      if ((int)threadIdx.x < half) parent = unite(lds[2 * threadIdx.x], lds[2 * threadIdx.x + 1]);
+      __syncthreads();
+      start >>= 1;
+      if ((int)threadIdx.x < half) {
+        lds[threadIdx.x] = parent;
+        nodes[start + threadIdx.x] = parent;
+      }
+      __syncthreads();
+      count = half;
     }
+    int top = bottom / width;  // nodes of the level this stage ended on
+    if (top <= 1) return;
+    int next_width = top < HIER_GROUP ? top : HIER_GROUP;
+    int next = group / next_width;
+    int *counter = counters + stage * HIER_GROUP + next;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    count = half;
+    if (threadIdx.x == 0) proceed = atomicAdd(counter, 1) == next_width - 1;
+    __syncthreads();
+    if (!proceed) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (threadIdx.x == 0) *counter = 0;
+    bottom = top;
+    group = next;
+    width = next_width;
   }
 }
 
@@ -435,12 +494,9 @@ void build_hierarchy(hipStream_t s, const Box *boxes, const int *count, int coun
              scratch);
   hipLaunchKernelGGL(h_leaves, dim3((h.size + 255) / 256), dim3(256), 0, s, boxes, count,
                      count_static, cap, h.ids, h.leaves, h.nodes);
-  int stage = 0;
-  for (long bottom = h.size; bottom > 1; bottom >>= 9, ++stage) {
-    long blocks = bottom / 512;
-    hipLaunchKernelGGL(h_levels, dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(256), 0, s, h.nodes,
-                       h.leaves, stage);
-  }
+  long blocks = h.size / HIER_GROUP;
+  hipLaunchKernelGGL(h_levels, dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(256), 0, s, h.nodes, h.leaves,
+                     h.level_counters);
 }
 
 }  // namespace peel

@@ -29,8 +29,12 @@ void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int
                 const int *count, int count_static, int capacity, int bit_begin, int bit_end,
                 const SortScratch &scratch);
 
+// Zero between scans: allocation clears it and every scan's last tile clears
+// what the scan used.
 struct ScanScratch {
-  int *block_sums = nullptr;  // [max_blocks]
+  unsigned long long *words = nullptr;  // [max_blocks] look-back words
+  int *counters = nullptr;              // [2] next tile ticket, finished tiles
+  int *status = nullptr;                // workspace status: a look-back that waits too long sets ERR_WAIT
   int max_blocks = 0;
 };
 
@@ -42,11 +46,15 @@ void scan_exclusive(hipStream_t s, const int *in, int *out, const int *count, in
 // the power of two at or above the live count (stored in *leaves), so a
 // captured graph builds and traverses only the live part of a pool whose
 // capacity is `size`.
+constexpr int HIER_GROUP = 512;                   // nodes reduced by one block per stage
+constexpr int HIER_COUNTERS = 4 * HIER_GROUP;     // arrival counters of up to four stages above the first
+
 struct Hierarchy {
   uint64_t *keys = nullptr, *alt_keys = nullptr;
   int *ids = nullptr, *alt_ids = nullptr;  // [size]
   Box *nodes = nullptr;                     // [2 * size]
   int *leaves = nullptr;                    // device: current leaf count
+  int *level_counters = nullptr;            // [HIER_COUNTERS] zero between builds
   int size = 1;                             // capacity (power of two)
   __device__ Tree tree() const { return Tree{nodes, ids, __hip_atomic_load(leaves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)}; }
 };
