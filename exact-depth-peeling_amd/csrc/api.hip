@@ -111,6 +111,7 @@ int peel::plan(Ws &w, const PeelDesc *d) {
   w.ray_lanes_target = 524288;
   w.heavy_lanes = 65536;
   w.long_edge_cells = 48;
+  w.bucket_log = 2;
   return 0;
 }
 
@@ -442,6 +443,7 @@ int peel_get_tuning(PeelWorkspace *ws, PeelTuning *t) {
   t->ray_lanes_target = w.ray_lanes_target;
   t->heavy_lanes = w.heavy_lanes;
   t->long_edge_cells = w.long_edge_cells;
+  t->leaf_bucket_log2 = w.bucket_log;
   t->max_blocks_per_cu = ws->max_per_cu;
   return 0;
 }
@@ -453,11 +455,13 @@ int peel_set_tuning(PeelWorkspace *ws, const PeelTuning *t) {
   if (t->ray_lanes_target < 64) return fail(2, "ray_lanes_target must be at least 64");
   if (t->heavy_lanes < 1024) return fail(2, "heavy_lanes must be at least 1024");
   if (t->long_edge_cells < 1) return fail(2, "long_edge_cells must be positive");
+  if (t->leaf_bucket_log2 < 0 || t->leaf_bucket_log2 > 4) return fail(2, "leaf_bucket_log2 must be in [0, 4]");
   Ws &w = ws->w;
   w.persistent_blocks = ws->cus * t->persistent_blocks_per_cu;
   w.ray_lanes_target = t->ray_lanes_target;
   w.heavy_lanes = t->heavy_lanes;
   w.long_edge_cells = t->long_edge_cells;
+  w.bucket_log = t->leaf_bucket_log2;
   // the captured graph holds the old launch shapes and kernel arguments
   if (ws->exec) {
     hipGraphExecDestroy(ws->exec);

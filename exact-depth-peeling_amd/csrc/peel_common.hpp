@@ -360,6 +360,111 @@ struct WideWalk {
     return -1;
   }
 };
+// Depth-first walk of a subtree without any stack in memory: the walk goes
+// down four grandchildren at a time and ends in leaf buckets of 2^bucket_log
+// leaves (bucket_log = the tuned bucket size or one more, so that the levels
+// in between pair up), whose boxes are tested together (one to eight 128-byte
+// lines). What is left to visit is one 4-bit mask of overlapping grandchildren
+// per level, packed in one 64-bit word (16 levels), plus the bucket's mask of
+// overlapping leaves; the node a popped mask bit stands for follows from the
+// current node by shifts. The leaves reached and their order are those of the
+// binary walk over the same subtree (a box lies inside its parent's), and the
+// binary continuation after a leaf is still successor(leaf).
+struct TrailWalk {
+  uint64_t trail;   // level k (grandchildren of a level k-1 node) at bits [4k, 4k + 4)
+  uint32_t bucket;  // overlapping leaves of the current bucket not yet returned
+  int cur;          // node of the deepest expanded level (kc)
+  int kc;           // its level (-1: the subtree root)
+  int root, lvls;   // subtree root; 4-ary levels between it and the bucket roots
+  int blog;         // log2 of the bucket size
+
+  // plan the levels of the subtree at `r` with buckets of about 2^want leaves
+  __device__ __forceinline__ void plan(const Tree &t, int r, int want) {
+    int D = 31 - __clz(t.leaves), h = D - (31 - __clz(r));
+    blog = h <= want ? h : want + ((h - want) & 1);
+    lvls = (h - blog) >> 1;
+    root = r;
+    trail = 0;
+    bucket = 0;
+  }
+  __device__ __forceinline__ uint32_t mask4(const Tree &t, int n, float4 lo, float4 hi) const {
+    const Box *g = t.nodes + 4 * n;
+    Box g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
+    return (uint32_t)overlaps(g0, lo, hi) | (uint32_t)overlaps(g1, lo, hi) << 1 |
+           (uint32_t)overlaps(g2, lo, hi) << 2 | (uint32_t)overlaps(g3, lo, hi) << 3;
+  }
+  __device__ __forceinline__ uint32_t leaves_mask(const Tree &t, int b, float4 lo, float4 hi) const {
+    const Box *g = t.nodes + ((long)b << blog);
+    uint32_t m = 0;
+    for (int j = 0; j < (1 << blog); j += 4) {
+      Box g0 = g[j], g1 = g[j + 1], g2 = g[j + 2], g3 = g[j + 3];
+      m |= ((uint32_t)overlaps(g0, lo, hi) | (uint32_t)overlaps(g1, lo, hi) << 1 |
+            (uint32_t)overlaps(g2, lo, hi) << 2 | (uint32_t)overlaps(g3, lo, hi) << 3) << j;
+    }
+    return m;
+  }
+  __device__ __forceinline__ uint32_t small_mask(const Tree &t, int b, float4 lo, float4 hi) const {
+    // buckets of one or two leaves (subtrees of height 0 or 1)
+    const Box *g = t.nodes + ((long)b << blog);
+    uint32_t m = overlaps(g[0], lo, hi);
+    if (blog == 1) m |= (uint32_t)overlaps(g[1], lo, hi) << 1;
+    return m;
+  }
+  __device__ __forceinline__ uint32_t bucket_mask(const Tree &t, int b, float4 lo, float4 hi) const {
+    return blog >= 2 ? leaves_mask(t, b, lo, hi) : small_mask(t, b, lo, hi);
+  }
+  // every leaf of the subtree at r
+  __device__ void start(const Tree &t, int r, int want, float4 lo, float4 hi) {
+    plan(t, r, want);
+    kc = -1;
+    cur = r;
+    if (lvls == 0) {
+      kc = 0;
+      bucket = bucket_mask(t, r, lo, hi);
+    } else {
+      trail = mask4(t, r, lo, hi);
+    }
+  }
+  // the leaves of the subtree at r from leaf node `first` on (first inside it)
+  __device__ void start_at(const Tree &t, int r, int first, int want, float4 lo, float4 hi) {
+    plan(t, r, want);
+    int b = first >> blog;  // its bucket root, at level lvls - 1 (or r itself)
+    for (int k = 0; k < lvls; ++k) {
+      int node = b >> (2 * (lvls - 1 - k));  // the path's node at level k
+      int idx = node & 3;
+      uint32_t later = mask4(t, node >> 2, lo, hi) & (0xEu << idx) & 0xFu;  // its right siblings
+      trail |= (uint64_t)later << (4 * k);
+    }
+    cur = b;
+    kc = lvls - 1;
+    uint32_t m = bucket_mask(t, b, lo, hi);
+    int from = first - (b << blog);
+    bucket = m & (0xFFFFFFFFu << from);
+  }
+  // next overlapping leaf: its id, and its heap node in `leaf_node`; -1 when done
+  __device__ int next(const Tree &t, float4 lo, float4 hi, int &leaf_node) {
+    while (true) {
+      if (bucket) {
+        int j = __ffs(bucket) - 1;
+        bucket &= bucket - 1;
+        leaf_node = (cur << blog) + j;
+        return t.ids[leaf_node - t.leaves];
+      }
+      if (!trail) return -1;
+      int lvl = (63 - __clzll((long long)trail)) >> 2;  // deepest level with a pending node
+      int idx = __ffs((uint32_t)(trail >> (4 * lvl)) & 0xFu) - 1;
+      trail &= ~(1ull << (4 * lvl + idx));
+      int parent = lvl == 0 ? root : cur >> (2 * (kc - lvl + 1));
+      cur = 4 * parent + idx;
+      kc = lvl;
+      if (kc == lvls - 1)
+        bucket = bucket_mask(t, cur, lo, hi);
+      else
+        trail |= (uint64_t)mask4(t, cur, lo, hi) << (4 * (kc + 1));
+    }
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Wave64 aggregation. Same-address atomics serialise in L2, so counters and
 // pool cursors are bumped once per wave.

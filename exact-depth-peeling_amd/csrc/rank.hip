@@ -499,8 +499,12 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
     float4 lo, hi;
     ray_box(x, lo, hi);
     const int root = K + sub;
-    int node = active ? root : 0;
+    int node = active ? root : 0;  // binary continuation, for hand-offs
     const int end = successor(root);
+    TrailWalk walk;
+    walk.trail = 0;
+    walk.bucket = 0;
+    if (active) walk.start(tree, root, w.bucket_log, lo, hi);
     int count = 0, visits = 0;
     bool handed = false, queue_full = false;
     while (node > 0 && node != end) {
@@ -517,8 +521,10 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
         }
         queue_full = true;
       }
-      int leaf = query_leaf(tree, lo, hi, node, end);
-      if (leaf < 0) break;
+      int leaf_node = 0;
+      if (walk.next(tree, lo, hi, leaf_node) < 0) break;
+      node = successor(leaf_node);
+      int leaf = leaf_node - tree.leaves;
       visits += 1;
       bool hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
       store_hits(w, hit, leaf, i, x);
@@ -564,26 +570,36 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   int filtered = 0, fallbacks = 0;
   for (long base = (long)blockIdx.x * blockDim.x; base < total; base += workers) {
     long task = base + threadIdx.x;
-    int node = 0, end = 0, i = 0;
+    int i = 0, part_root = 0, from = 0;
     if (task < total) {
       int4 job = w.heavy[task / parts];
       int part = (int)(task % parts);
-      int part_root = job.z * parts + part;
-      int first = part_root * width;  // first leaf node of the part
+      int root = job.z * parts + part;
+      int first = root * width;  // first leaf node of the part
       if (job.y < first + width) {
-        node = job.y > first ? job.y : part_root;
-        end = successor(part_root);
+        part_root = root;
+        from = job.y > first ? job.y : 0;  // the leaf the job continues from, inside this part
         i = job.x;
       }
     }
-    double2 x = node > 0 ? w.samples[i] : make_double2(0, 0);
+    double2 x = part_root > 0 ? w.samples[i] : make_double2(0, 0);
     float4 lo, hi;
     ray_box(x, lo, hi);
+    TrailWalk walk;
+    walk.trail = 0;
+    walk.bucket = 0;
+    if (part_root > 0) {
+      if (from)
+        walk.start_at(tree, part_root, from, w.bucket_log, lo, hi);
+      else
+        walk.start(tree, part_root, w.bucket_log, lo, hi);
+    }
     int count = 0;
     float2 x32 = make_float2((float)x.x, (float)x.y);
-    while (node > 0 && node != end) {
-      int leaf = query_leaf(tree, lo, hi, node, end);
-      if (leaf < 0) break;
+    while (part_root > 0) {
+      int leaf_node = 0;
+      if (walk.next(tree, lo, hi, leaf_node) < 0) break;
+      int leaf = leaf_node - tree.leaves;
       visits += 1;
       bool hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
       store_hits(w, hit, leaf, i, x);

@@ -167,6 +167,7 @@ struct Ws {
   int ray_lanes_target;
   int heavy_lanes;
   int long_edge_cells;
+  int bucket_log;           // leaf buckets of the ray walks: 2^bucket_log (or twice) leaves
   // hit groups above 2,048 are sorted one phase per launch instead of by one
   // launch with grid barriers (set for graphs whose members run concurrently)
   int split_huge;

@@ -89,6 +89,7 @@ class PeelTuning(ctypes.Structure):
         ("ray_lanes_target", ctypes.c_int32),
         ("heavy_lanes", ctypes.c_int32),
         ("long_edge_cells", ctypes.c_int32),
+        ("leaf_bucket_log2", ctypes.c_int32),
         ("max_blocks_per_cu", ctypes.c_int32),
     ]
 

@@ -15,7 +15,10 @@ Fields (``include/peel355.h`` ``PeelTuning``):
   (64 lanes per ray) before rays fall back to one lane each;
 * ``heavy_lanes``: lane budget of the subtree jobs of heavy rays and beams;
 * ``long_edge_cells``: grid cells beyond which a polygon edge is noded by a
-  whole wave instead of one lane.
+  whole wave instead of one lane;
+* ``leaf_bucket_log2``: the ray walks stop descending 2^n (or 2^(n+1)) leaves
+  above the bottom and test those leaves' boxes together (the hierarchy's
+  leaf size as the walks see it).
 """
 
 import json
@@ -28,6 +31,7 @@ CANDIDATES = {
     "ray_lanes_target": (65536, 131072, 262144, 524288, 1048576, 4194304),
     "heavy_lanes": (16384, 32768, 65536, 131072, 262144),
     "long_edge_cells": (16, 32, 48, 96, 192),
+    "leaf_bucket_log2": (1, 2, 3, 4),
 }
 TUNABLE = tuple(CANDIDATES)
 

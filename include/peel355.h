@@ -133,6 +133,7 @@ typedef struct PeelTuning {
   int32_t ray_lanes_target;         /* lanes in flight allowed at 64 lanes per ray (else 1 lane per ray), >= 64 */
   int32_t heavy_lanes;              /* lane budget of the heavy-ray and beam subtree jobs, >= 1024 */
   int32_t long_edge_cells;          /* grid cells beyond which a polygon edge is noded by a wave, >= 1 */
+  int32_t leaf_bucket_log2;         /* ray walks test 2^n (or 2^(n+1)) hierarchy leaves together, 0..4 */
   int32_t max_blocks_per_cu;        /* read only: resident k_subdivide blocks per CU */
 } PeelTuning;
 int peel_get_tuning(PeelWorkspace *ws, PeelTuning *tuning);

@@ -20,8 +20,10 @@ def test_tuning_arguments():
     t = p.tuning
     assert t["persistent_blocks_per_cu"] == 1 and t["max_blocks_per_cu"] >= 1
     assert t["ray_lanes_target"] == 524288 and t["heavy_lanes"] == 65536 and t["long_edge_cells"] == 48
+    assert t["leaf_bucket_log2"] == 2
     for bad in ({"persistent_blocks_per_cu": 0}, {"persistent_blocks_per_cu": t["max_blocks_per_cu"] + 1},
-                {"ray_lanes_target": 8}, {"heavy_lanes": 10}, {"long_edge_cells": 0}):
+                {"ray_lanes_target": 8}, {"heavy_lanes": 10}, {"long_edge_cells": 0}, {"leaf_bucket_log2": 5},
+                {"leaf_bucket_log2": -1}):
         with pytest.raises(ValueError):
             p.set_tuning(**bad)
     with pytest.raises(ValueError):
@@ -38,9 +40,9 @@ def test_outputs_do_not_depend_on_tuning(filter_fp32):
     want = (canonical_patches(r0, s0, l0), canonical_triangles(uv0, w0, ts0, tl0))
     limit = p.tuning["max_blocks_per_cu"]
     for setting in ({"persistent_blocks_per_cu": limit, "ray_lanes_target": 64, "heavy_lanes": 1024,
-                     "long_edge_cells": 1},
+                     "long_edge_cells": 1, "leaf_bucket_log2": 0},
                     {"persistent_blocks_per_cu": 1, "ray_lanes_target": 1 << 30, "heavy_lanes": 1 << 24,
-                     "long_edge_cells": 1 << 20}):
+                     "long_edge_cells": 1 << 20, "leaf_bucket_log2": 4}):
         p.set_tuning(**setting)
         p.run()
         _, r, s, l, uv, w, ts, tl = gpu_outputs(p)
@@ -54,7 +56,8 @@ def test_autotune_and_cache(tmp_path):
     p = knot_peeler(128, 64)
     report = tune.autotune(p, replays=3, rounds=1,
                            candidates={"persistent_blocks_per_cu": (1, 2), "ray_lanes_target": (131072, 524288),
-                                       "heavy_lanes": (32768, 65536), "long_edge_cells": (32, 48)})
+                                       "heavy_lanes": (32768, 65536), "long_edge_cells": (32, 48),
+                                       "leaf_bucket_log2": (2, 3)})
     assert report["best_ms"] <= report["default_ms"]
     assert {k: p.tuning[k] for k in tune.TUNABLE} == report["best"]
     cache = tune.TuningCache(tmp_path / "t.json")
