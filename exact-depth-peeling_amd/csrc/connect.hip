@@ -193,10 +193,12 @@ __device__ void connect_one(const Ws &w, int i, int n) {
       float4 lo = make_float4((float)x.x - 2.0e-7f, (float)x.y - 2.0e-7f, -1.0e30f, 0.f);
       float4 hi = make_float4((float)x.x + 2.0e-7f, (float)x.y + 2.0e-7f, (float)(z + pad), 0.f);
       Tree tree = w.contour_tree.tree();
-      int node = 1;
+      TrailWalk walk;
+      walk.start(tree, 1, w.bucket_log, lo, hi);
       bool safe = true;
-      while (node > 0) {
-        int c = query_next(tree, lo, hi, node);
+      while (true) {
+        int leaf_node = 0;
+        int c = walk.next(tree, lo, hi, leaf_node);
         if (c < 0) break;
         visits += 1;
         double2 u = w.cseg_a[c], v = w.cseg_b[c];

@@ -234,14 +234,22 @@ __global__ __launch_bounds__(256) void k_certify(Ws w) {
     // identical samples already share their exact group
     ok = !(corner[0].x == corner[3].x && corner[0].y == corner[3].y);
   }
-  int node = ok ? 1 : 0, visits = 0;
+  int node = ok ? 1 : 0, visits = 0;  // binary continuation, for the hand-off
   bool accepted = ok;
   Tree tree = w.source_tree.tree();
   float4 lo, hi;
-  if (ok) beam_box(w, c, lo, hi);
+  TrailWalk walk;
+  walk.trail = 0;
+  walk.bucket = 0;
+  if (ok) {
+    beam_box(w, c, lo, hi);
+    walk.start(tree, 1, w.bucket_log, lo, hi);
+  }
   for (int step = 0; step < 32 && node > 0; ++step) {
-    int leaf = query_leaf(tree, lo, hi, node);
-    if (leaf < 0) break;
+    int leaf_node = 0;
+    if (walk.next(tree, lo, hi, leaf_node) < 0) break;
+    node = successor(leaf_node);
+    int leaf = leaf_node - tree.leaves;
     visits += 1;
     if (!covers_leaf(w, leaf, corner)) {
       accepted = false;
@@ -276,15 +284,19 @@ __global__ __launch_bounds__(256) void k_certify_heavy(Ws w, int workers) {
     int first = tree.leaves + part * width;
     if (job.y >= first + width || load_i(w.beam_valid + c) == 0) continue;
     int root = parts + part;
-    int node = job.y > first ? job.y : root;
-    int end = successor(root);
     double2 corner[4];
     for (int k = 0; k < 4; ++k) corner[k] = beam_corner(w, c, k);
     float4 lo, hi;
     beam_box(w, c, lo, hi);
-    while (node > 0 && node != end) {
-      int leaf = query_leaf(tree, lo, hi, node, end);
-      if (leaf < 0) break;
+    TrailWalk walk;
+    if (job.y > first)
+      walk.start_at(tree, root, job.y, w.bucket_log, lo, hi);
+    else
+      walk.start(tree, root, w.bucket_log, lo, hi);
+    while (true) {
+      int leaf_node = 0;
+      if (walk.next(tree, lo, hi, leaf_node) < 0) break;
+      int leaf = leaf_node - tree.leaves;
       visits += 1;
       if (!covers_leaf(w, leaf, corner)) {
         atomicExch(w.beam_valid + c, 0);
