@@ -291,75 +291,6 @@ __device__ __forceinline__ int query_next(const Tree &t, float4 lo, float4 hi, i
   return -1;
 }
 
-// Depth-first walk of the same tree, one 128-byte line per dependent load: an
-// overlapping internal node whose grandchildren are internal or leaves loads
-// its four grandchildren together (nodes 4n .. 4n+3 are adjacent, 32 bytes
-// each), tests them and pushes the overlapping ones; subtrees still to visit
-// wait on a per-lane stack instead of being found again by climbing. A
-// grandchild box lies inside its parent's, so skipping the middle level
-// prunes nothing the binary walk keeps: the leaves reached, and their order,
-// are query_next's. The stack is rebuilt from a binary continuation node
-// (start) with integer arithmetic only, and the continuation after a leaf is
-// still successor(leaf), so tasks keep storing one binary cursor.
-//
-// Stack entries: node << 1 | known, known = its box was tested and overlaps.
-// The stack holds at most one pending subtree per level above the deepest
-// expansion and three per pair of levels expanded, about 1.5 entries per
-// level: WALK_STACK covers the deepest tree a workspace allocates (2^30
-// leaves).
-constexpr int WALK_STACK = 48;
-struct WideWalk {
-  int *slot;    // this lane's entry 0; entry k at slot[k * stride]
-  int stride;
-  int top;
-  __device__ __forceinline__ void push(int e) { slot[top++ * stride] = e; }
-  // pending subtrees of the binary walk continuing at `node`, in visit order:
-  // node itself, then the right sibling of every ancestor-or-self reached by
-  // climbing out of a left child
-  __device__ void start(int node) {
-    top = 0;
-    if (node <= 0) return;
-    push(node << 1);
-    int y = node;
-    while (true) {
-      while (y > 1 && (y & 1)) y >>= 1;
-      if (y <= 1) break;
-      y += 1;
-      push(y << 1);
-    }
-    // pushed in visit order: reverse so that `node` is on top
-    for (int i = 0, j = top - 1; i < j; ++i, --j) {
-      int a = slot[i * stride];
-      slot[i * stride] = slot[j * stride];
-      slot[j * stride] = a;
-    }
-  }
-  // next overlapping leaf: its id, and its heap node in `leaf_node`; -1 when done
-  __device__ int next(const Tree &t, float4 lo, float4 hi, int &leaf_node) {
-    while (top > 0) {
-      int e = slot[--top * stride];
-      int n = e >> 1;
-      if (!(e & 1) && !overlaps(t.nodes[n], lo, hi)) continue;
-      if (n >= t.leaves) {
-        leaf_node = n;
-        return t.ids[n - t.leaves];
-      }
-      if (2 * n >= t.leaves) {  // children are leaves: one 64-byte pair
-        Box b0 = t.nodes[2 * n], b1 = t.nodes[2 * n + 1];
-        if (overlaps(b1, lo, hi)) push(((2 * n + 1) << 1) | 1);
-        if (overlaps(b0, lo, hi)) push((2 * n) << 1 | 1);
-      } else {  // four grandchildren in one line
-        const Box *g = t.nodes + 4 * n;
-        Box g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
-        if (overlaps(g3, lo, hi)) push(((4 * n + 3) << 1) | 1);
-        if (overlaps(g2, lo, hi)) push(((4 * n + 2) << 1) | 1);
-        if (overlaps(g1, lo, hi)) push(((4 * n + 1) << 1) | 1);
-        if (overlaps(g0, lo, hi)) push(((4 * n) << 1) | 1);
-      }
-    }
-    return -1;
-  }
-};
 // Depth-first walk of a subtree without any stack in memory: the walk goes
 // down four grandchildren at a time and ends in leaf buckets of 2^bucket_log
 // leaves (bucket_log = the tuned bucket size or one more, so that the levels

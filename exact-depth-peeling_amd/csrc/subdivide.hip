@@ -52,8 +52,6 @@ __device__ __forceinline__ double2 vtx(const Slot &s, int j) { return s.xy[(long
 constexpr int SUB_THREADS = 256;
 constexpr int SUB_LDS = 12;
 __shared__ double2 sub_ring_lds[SUB_LDS * SUB_THREADS];
-// each lane's hierarchy walk stack (WideWalk), entry k at [k * SUB_THREADS + lane]
-__shared__ int sub_walk_lds[WALK_STACK * SUB_THREADS];
 // kept out of line so the compiler does not fold the LDS and global loads
 // into one generic load
 __device__ __noinline__ double2 slot_vertex(const double2 *slot, long stride, int j) {
@@ -145,7 +143,8 @@ __device__ void save_output(const Ws &w, const LdsRing &s, int n, int source, in
 
 __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
   LdsRing ring{(int)threadIdx.x, nullptr, 0};
-  WideWalk walk{sub_walk_lds + threadIdx.x, SUB_THREADS, 0};
+  TrailWalk walk;
+  bool fresh = false;  // the task's walk starts at its first visit, from its binary cursor
   const int nf = w.nf;
   const Tree tree = w.contour_tree.tree();
   int lane = threadIdx.x & 63;
@@ -197,7 +196,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
           for (int j = 0; j < n && j < SUB_LDS; ++j) ring.put(j, vtx(src, j));
           ring.slot = src.xy;
           ring.stride = src.stride;
-          walk.start(node);
+          fresh = true;
         }
       }
     }
@@ -232,6 +231,17 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
     float4 lo, hi;
     pad_box(lx, ly, lz, hx, hy, hz, lo, hi);
     lo.z = -1e30f;
+    if (fresh) {
+      // the leaves still to visit are those from the leftmost leaf under the
+      // cursor on (a task continues after the candidate that cut its parent)
+      fresh = false;
+      walk.trail = 0;
+      walk.bucket = 0;
+      if (node > 0) {
+        int first = node << (__clz(node) - __clz(tree.leaves));
+        walk.start_at(tree, 1, first, w.bucket_log, lo, hi);
+      }
+    }
     bool done = false, cut = false;
     for (int step = 0; step < 32; ++step) {
       int leaf = 0;
