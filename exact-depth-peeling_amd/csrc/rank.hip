@@ -370,17 +370,23 @@ __global__ void k_leaf_sources(Ws w) {
 // Half-open point-in-convex test (event_kernels.py:131-148): a point on an
 // edge belongs to the ring only when the edge is "owned" (b.y > a.y, or
 // b.y == a.y and b.x < a.x), so a tessellated sheet counts once.
+// The whole 128-byte record is loaded with the count, so the test waits on
+// one round trip instead of the count and then the vertices.
 __device__ __forceinline__ bool contains_leaf(const Ws &w, int k, double2 x) {
-  int n = w.leaf_n[k];
   const double2 *r = w.leaf_xy + 8L * k;
+  double2 v[SRCV];
+#pragma unroll
+  for (int j = 0; j < SRCV; ++j) v[j] = r[j];
+  int n = w.leaf_n[k];
   bool inside = n > 0;
-  double2 a = r[0];
-  for (int j = 0; j < n && inside; ++j) {
-    double2 b = r[(j + 1) % n];
-    double s = cross3(a, b, x);
-    bool owned = b.y > a.y || (b.y == a.y && b.x < a.x);
-    if (s < 0.0 || (s == 0.0 && !owned)) inside = false;
-    a = b;
+#pragma unroll
+  for (int j = 0; j < SRCV; ++j) {
+    if (j < n && inside) {
+      double2 a = v[j], b = j + 1 < n ? v[(j + 1) % SRCV] : v[0];
+      double s = cross3(a, b, x);
+      bool owned = b.y > a.y || (b.y == a.y && b.x < a.x);
+      if (s < 0.0 || (s == 0.0 && !owned)) inside = false;
+    }
   }
   return inside;
 }
@@ -392,13 +398,17 @@ __device__ __forceinline__ bool contains_leaf(const Ws &w, int k, double2 x) {
 // contains_leaf's for every edge.
 __device__ __forceinline__ bool contains_leaf32(const Ws &w, int k, double2 x, float2 x32, int &filtered,
                                                 int &fallbacks) {
-  int n = w.leaf_n[k];
   const float2 *r = w.leaf_xy32 + 8L * k;
+  float2 v[SRCV];
+#pragma unroll
+  for (int j = 0; j < SRCV; ++j) v[j] = r[j];
+  int n = w.leaf_n[k];
   const float tol = w.leaf_tol[k];
   bool inside = n > 0;
-  float2 a = r[0];
-  for (int j = 0; j < n && inside; ++j) {
-    float2 b = r[(j + 1) % n];
+#pragma unroll
+  for (int j = 0; j < SRCV; ++j) {
+    if (!(j < n && inside)) continue;
+    float2 a = v[j], b = j + 1 < n ? v[(j + 1) % SRCV] : v[0];
     float s32 = (b.x - a.x) * (x32.y - a.y) - (b.y - a.y) * (x32.x - a.x);
     if (s32 < -tol) {
       inside = false;
@@ -412,7 +422,6 @@ __device__ __forceinline__ bool contains_leaf32(const Ws &w, int k, double2 x, f
       bool owned = B.y > A.y || (B.y == A.y && B.x < A.x);
       if (s < 0.0 || (s == 0.0 && !owned)) inside = false;
     }
-    a = b;
   }
   return inside;
 }
@@ -422,17 +431,55 @@ __device__ __forceinline__ bool contains_any(const Ws &w, int k, double2 x, floa
   return w.filter32 ? contains_leaf32(w, k, x, x32, filtered, fallbacks) : contains_leaf(w, k, x);
 }
 
-// store_hit (event_kernels.py:151-169) with one pool-cursor atomic per wave.
-__device__ __forceinline__ void store_hits(const Ws &w, bool hit, int leaf, int group, double2 x) {
-  long long slot = wave_claim((unsigned long long *)&w.metrics[M_RAY_HITS], hit);
-  if (!hit) return;
-  if (slot < (long long)w.cap.hits) {
-    w.hit_depth[slot] = value(w.leaf_score[leaf], x);
-    w.hit_src[slot] = w.source_tree.ids[leaf];
-    w.hit_group[slot] = group;
-  } else {
-    set_error(w.status, ERR_HITS);
+// store_hit (event_kernels.py:151-169), staged: a wave collects its hits in
+// LDS and claims pool slots for them with one atomic per 64 or more. (A claim
+// per iteration made every ray walk wait on the one pool counter, whose
+// atomics serialise: 5-20 us per candidate on the torus knot and the shells.)
+constexpr int HIT_STAGE = 128;  // a flush starts at 64 staged hits, a step adds at most 64
+struct HitStage {
+  volatile double *depth;
+  volatile int *src, *group;
+  int staged;  // wave-uniform: every lane calls with the same history
+};
+__shared__ double hit_stage_depth[4][HIT_STAGE];
+__shared__ int hit_stage_src[4][HIT_STAGE], hit_stage_group[4][HIT_STAGE];
+
+__device__ __forceinline__ HitStage hit_stage() {
+  int wave = threadIdx.x >> 6;
+  return HitStage{hit_stage_depth[wave], hit_stage_src[wave], hit_stage_group[wave], 0};
+}
+__device__ void flush_hits(const Ws &w, HitStage &st) {
+  if (st.staged == 0) return;
+  __builtin_amdgcn_wave_barrier();
+  const int lane = threadIdx.x & 63;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd((unsigned long long *)&w.metrics[M_RAY_HITS], (unsigned long long)st.staged);
+  base = __shfl(base, 0, 64);
+  for (int j = lane; j < st.staged; j += 64) {
+    unsigned long long slot = base + j;
+    if (slot < (unsigned long long)w.cap.hits) {
+      w.hit_depth[slot] = st.depth[j];
+      w.hit_src[slot] = st.src[j];
+      w.hit_group[slot] = st.group[j];
+    } else {
+      set_error(w.status, ERR_HITS);
+    }
   }
+  __builtin_amdgcn_wave_barrier();
+  st.staged = 0;
+}
+// Called by every lane of the wave together (wave-uniform control flow).
+__device__ __forceinline__ void stage_hits(const Ws &w, HitStage &st, bool hit, int leaf, int group, double2 x) {
+  uint64_t m = __ballot(hit);
+  if (!m) return;
+  if (hit) {
+    int pos = st.staged + __popcll(m & lanemask_lt());
+    st.depth[pos] = value(w.leaf_score[leaf], x);
+    st.src[pos] = w.source_tree.ids[leaf];
+    st.group[pos] = group;
+  }
+  st.staged += __popcll(m);
+  if (st.staged >= 64) flush_hits(w, st);
 }
 
 __device__ __forceinline__ void ray_box(double2 x, float4 &lo, float4 &hi) {
@@ -493,6 +540,7 @@ __global__ void k_ray_keys(Ws w) {
 // entries of the Morton-sorted ray order, K lanes each.
 __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
   if (w.status[ST_ERR] != 0) return;
+  HitStage stage = hit_stage();
   const int n = load_i(w.control + C_LEADERS);
   const int lane = threadIdx.x & 63;
   const Tree tree = w.source_tree.tree();
@@ -519,8 +567,11 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
     if (active) walk.start(tree, root, w.bucket_log, lo, hi);
     int count = 0, visits = 0;
     bool handed = false, queue_full = false;
-    while (node > 0 && node != end) {
-      if (visits >= HANDOFF && !queue_full) {
+    bool live = node > 0 && node != end;
+    while (__ballot(live)) {  // wave-uniform: the hit stage needs every lane
+      bool hit = false;
+      int leaf = 0;
+      if (live && visits >= HANDOFF && !queue_full) {
         // the unvisited suffix of this subtree starts at the first leaf
         // below the cursor
         int first = node;
@@ -529,18 +580,25 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
         if (slot < w.cap.regions) {
           w.heavy[slot] = make_int4(i, first, root, 0);
           handed = true;
-          break;
+          live = false;
+        } else {
+          queue_full = true;
         }
-        queue_full = true;
       }
-      int leaf_node = 0;
-      if (walk.next(tree, lo, hi, leaf_node) < 0) break;
-      node = successor(leaf_node);
-      int leaf = leaf_node - tree.leaves;
-      visits += 1;
-      bool hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
-      store_hits(w, hit, leaf, i, x);
-      count += hit;
+      if (live) {
+        int leaf_node = 0;
+        if (walk.next(tree, lo, hi, leaf_node) < 0) {
+          live = false;
+        } else {
+          node = successor(leaf_node);
+          leaf = leaf_node - tree.leaves;
+          visits += 1;
+          hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
+          count += hit;
+          live = node > 0 && node != end;
+        }
+      }
+      stage_hits(w, stage, hit, leaf, i, x);
     }
     long long total = group_sum(count, K);
     long long tested = group_sum(visits, K);
@@ -552,6 +610,7 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
       heavy += (any_handed > 0 || tested > 32) ? 1 : 0;
     }
   }
+  flush_hits(w, stage);
   wave_add_metric(&w.metrics[M_RAY_GROUPS], groups);
   wave_add_metric(&w.metrics[M_RAY_TESTS], visits_all);
   wave_add_metric(&w.metrics[M_HEAVY], heavy);
@@ -567,6 +626,7 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
 // and the part holding it starts there.
 __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   bool ok = w.status[ST_ERR] == 0;
+  HitStage stage = hit_stage();
   Tree tree = w.source_tree.tree();
   int jobs = ok ? load_i(w.control + C_HEAVY) : 0;
   if (jobs > w.cap.regions) jobs = w.cap.regions;
@@ -608,14 +668,22 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
     }
     int count = 0;
     float2 x32 = make_float2((float)x.x, (float)x.y);
-    while (part_root > 0) {
-      int leaf_node = 0;
-      if (walk.next(tree, lo, hi, leaf_node) < 0) break;
-      int leaf = leaf_node - tree.leaves;
-      visits += 1;
-      bool hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
-      store_hits(w, hit, leaf, i, x);
-      count += hit;
+    bool live = part_root > 0;
+    while (__ballot(live)) {  // wave-uniform: the hit stage needs every lane
+      bool hit = false;
+      int leaf = 0;
+      if (live) {
+        int leaf_node = 0;
+        if (walk.next(tree, lo, hi, leaf_node) < 0) {
+          live = false;
+        } else {
+          leaf = leaf_node - tree.leaves;
+          visits += 1;
+          hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
+          count += hit;
+        }
+      }
+      stage_hits(w, stage, hit, leaf, i, x);
     }
     // Consecutive tasks are parts of one job, so the lanes of a wave usually
     // add to one ray's count: those that share the first contributing lane's
@@ -632,6 +700,7 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
       if (count != 0 && !same) atomicAdd(&w.group_counts[i], count);
     }
   }
+  flush_hits(w, stage);
   wave_add_metric(&w.metrics[M_RAY_TESTS], visits);
   if (w.filter32) {
     wave_add_metric(&w.metrics[M_FILTER_TESTS], filtered);
