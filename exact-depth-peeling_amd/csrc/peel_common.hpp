@@ -95,7 +95,6 @@ enum Control : int {
   C_HEAD = 32,      // next ticket
   C_PENDING = 64,   // live frontier tasks
   C_HEAVY = 96,     // heavy ray jobs
-  C_NHITS = 128,    // stored hits (clamped to capacity)
   C_NBV = 129,      // boundary vertices (scan total)
   C_MEDIUM = 131,   // groups sorted by a workgroup
   C_HUGE = 132,     // groups too large for one LDS sort (listed from the end of `medium`)
@@ -110,7 +109,9 @@ enum Control : int {
   C_GRID = 142,      // cells per axis of the boundary-vertex grid
   C_LIVE = 143,      // patches left after convex coalescing
   C_SORTN = 144,     // layered route: residual pieces to order (0 once the loop ends)
+  C_RAYSORT = 145,   // leaders to put in Morton order (0 when a wave walks each ray)
   C_HUGE_SYNC = 160, // arrivals at the grid barriers of the huge-group sort
+  C_RAYKEY_DONE = 192, // blocks of k_ray_keys that have finished
   C_COUNT = 256,
 };
 

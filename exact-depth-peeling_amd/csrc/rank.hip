@@ -515,25 +515,62 @@ __device__ __forceinline__ long long group_sum(long long v, int k) {
   return v;
 }
 
-// Ray leaders in Morton order of their samples, so the lanes of a wave walk
-// the same hierarchy nodes (the reference sorts its queries by coordinates,
-// events.py:645-678).
-__global__ void k_ray_keys(Ws w) {
-  bool ok = w.status[ST_ERR] == 0;
-  long long leaders = 0;
-  ITEM_LOOP(i, npolys(w)) {
-    // 30-bit keys for leaders, all ones for the rest: leaders sort first
-    uint64_t key = ~0ull;
-    if (ok && w.groups[i] == i) {
-      double2 x = w.samples[i];
-      key = morton2((float)x.x, (float)x.y);
-      leaders += 1;
+// Ray leaders, compacted, with the Morton codes of their samples. When rays
+// take one lane each, the leaders are then sorted by code, so the lanes of a
+// wave walk the same hierarchy nodes (the reference sorts its queries by
+// coordinates, events.py:645-678); when a wave walks each ray (lanes over
+// leaf subtrees) the order does not matter and the sort is skipped: the last
+// block to finish sets the sort's count to the leaders or to zero. A block
+// claims slots for the leaders among 1,024 patches with one atomic (returning
+// atomics on one word serialise at about 15 ns each: one per wave cost the
+// shells 0.19 ms).
+constexpr int KEY_ITEMS = 4;
+__global__ __launch_bounds__(256) void k_ray_keys(Ws w) {
+  __shared__ int last, block_base, wave_total[4];
+  const bool ok = w.status[ST_ERR] == 0;
+  const int P = npolys(w), wave = threadIdx.x >> 6;
+  for (long base = (long)blockIdx.x * 256 * KEY_ITEMS; base < P; base += (long)gridDim.x * 256 * KEY_ITEMS) {
+    bool lead[KEY_ITEMS];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < KEY_ITEMS; ++k) {
+      long i = base + threadIdx.x + 256L * k;
+      lead[k] = ok && i < P && w.groups[i] == (int)i;
+      mine += lead[k];
     }
-    w.ray_keys[i] = key;
-    w.ray_ids[i] = i;
+    int in_wave;
+    int before = wave_prefix(mine, in_wave);
+    if ((threadIdx.x & 63) == 0) wave_total[wave] = in_wave;
+    __syncthreads();
+    int all = 0;
+    for (int q = 0; q < 4; ++q) {
+      if (q < wave) before += wave_total[q];
+      all += wave_total[q];
+    }
+    if (threadIdx.x == 0) block_base = all ? atomicAdd(w.control + C_LEADERS, all) : 0;
+    __syncthreads();
+    int slot = block_base + before;
+#pragma unroll
+    for (int k = 0; k < KEY_ITEMS; ++k) {
+      if (!lead[k]) continue;
+      long i = base + threadIdx.x + 256L * k;
+      double2 x = w.samples[i];
+      w.ray_keys[slot] = morton2((float)x.x, (float)x.y);
+      w.ray_ids[slot] = (int)i;
+      ++slot;
+    }
+    __syncthreads();
   }
-  leaders = wave_sum(leaders);
-  if ((threadIdx.x & 63) == 0 && leaders) atomicAdd(w.control + C_LEADERS, (int)leaders);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(w.control + C_RAYKEY_DONE, 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    int leaders = load_i(w.control + C_LEADERS);
+    w.control[C_RAYSORT] = ray_lanes(w.source_tree.tree(), leaders, w.ray_lanes_target) == 1 ? leaders : 0;
+    w.control[C_RAYKEY_DONE] = 0;
+  }
 }
 
 // ray_hits (event_kernels.py:196-245), leaders only: the first C_LEADERS
@@ -1166,8 +1203,11 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
     hipLaunchKernelGGL(k_certify_heavy, dim3(32), dim3(256), 0, s, w, 32 * 256);
     hipLaunchKernelGGL(k_beam_apply, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   }
-  hipLaunchKernelGGL(k_ray_keys, stride_grid(w.cap.regions), dim3(256), 0, s, w);
-  radix_sort(s, w.ray_keys, w.ray_ids, w.ray_keys_alt, w.ray_ids_alt, w.status + ST_POLYS, 0,
+  {
+    long kg = ((long)w.cap.regions + 256 * KEY_ITEMS - 1) / (256 * KEY_ITEMS);
+    hipLaunchKernelGGL(k_ray_keys, dim3((unsigned)std::max(1L, std::min(kg, 1024L))), dim3(256), 0, s, w);
+  }
+  radix_sort(s, w.ray_keys, w.ray_ids, w.ray_keys_alt, w.ray_ids_alt, w.control + C_RAYSORT, 0,
              w.cap.regions, 0, 32, w.sort);
   mark(t, s, "samples_groups");
   hipLaunchKernelGGL(k_ray_hits, stride_grid(w.cap.regions), dim3(256), 0, s, w);
