@@ -330,7 +330,12 @@ __device__ __forceinline__ bool edge_setup(const Ws &w, int j, EdgeQuery &q, dou
 __device__ void mesh_edge_one(const Ws &w, int j, int n);
 
 __global__ __launch_bounds__(256) void k_mesh_edges(Ws w) {
-  if (w.status[ST_ERR] != 0) return;
+  // the whole block leaves or stays (mesh_edge_one synchronises the block;
+  // an error bit set by another block meanwhile is seen by one read only)
+  __shared__ int failed;
+  if (threadIdx.x == 0) failed = load_i(w.status + ST_ERR);
+  __syncthreads();
+  if (failed) return;
   int n = load_i(w.status + ST_BVERTS);
   WAVE_LOOP(base, n) mesh_edge_one(w, base + threadIdx.x, n);
 }
@@ -387,11 +392,22 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
   int source = valid ? w.out_src[i] : 0;
   double4 sc = valid ? w.scores[source] : make_double4(0, 0, 0, 0);
   int ntri = valid ? fan(w, q, center, count, reg, false, 0, 0, 0, sc) : 0;
+  // one slot claim per block (returning atomics on one word serialise)
+  __shared__ int wave_tris[4], block_base;
+  const int wave = threadIdx.x >> 6;
   int total;
   int offset = wave_prefix(ntri, total);
-  int base = 0;
-  if ((threadIdx.x & 63) == 0 && total > 0) base = atomicAdd(w.status + ST_TRIS, total);
-  base = __shfl(base, 0, 64);
+  if ((threadIdx.x & 63) == 0) wave_tris[wave] = total;
+  __syncthreads();
+  int all = 0;
+  for (int k = 0; k < 4; ++k) {
+    if (k < wave) offset += wave_tris[k];
+    all += wave_tris[k];
+  }
+  if (threadIdx.x == 0) block_base = all ? atomicAdd(w.status + ST_TRIS, all) : 0;
+  __syncthreads();
+  int base = block_base;
+  __syncthreads();  // the next edge of this block reuses the words
   if (!valid || ntri == 0) return;
   fan(w, q, center, count, reg, true, base + offset, source, layer, sc);
 }
