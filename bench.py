@@ -467,7 +467,7 @@ def main():
             "faces": len(host[1]),
             "vertices": len(host[0]),
             "route": "events",
-            "predicates": "fp32-filtered rays, fp64 fallback",
+            "predicates": "fp32-filtered containment, straddle, front_segment and depth order; fp64 fallback",
             "polygons": info["polygons"],
             "triangles": info["triangles"],
             "layers": info["layers"],
