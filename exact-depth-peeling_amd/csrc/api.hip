@@ -120,7 +120,7 @@ static void carve_hierarchy(Carver &c, Hierarchy &h) {
   c.take(h.alt_keys, h.size);
   c.take(h.ids, h.size);
   c.take(h.alt_ids, h.size);
-  c.take(h.nodes, 2L * h.size);
+  c.take(h.nodes, 2L * h.size + 4);  // four boxes of slack for group reads
   c.take(h.leaves, 1);
   c.take(h.level_counters, HIER_COUNTERS);
 }
@@ -158,6 +158,9 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
   c.take(w.seg_a, w.nseg);
   c.take(w.seg_b, w.nseg);
   c.take(w.seg_near, 2L * w.nseg);
+  w.cl_a = w.seg_a;
+  w.cl_b = w.seg_b;
+  w.cl_near = reinterpret_cast<double2 *>(w.seg_near);
   c.take(w.seg_rep, w.nseg);
   c.take(w.shash, w.shash_size);
   c.take(w.seg_boxes, w.nseg);
@@ -202,6 +205,7 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
   c.take(w.fr_node, F);
   c.take(w.fr_gen, F);
   c.take(w.fr_ready, F);
+  c.take(w.sub_partials, layered ? 1L : (long)SUB_PARTIAL_WAVES * SUB_COUNTERS);
   c.take(w.samples, R);
   c.take(w.reps, R);
   c.take(w.qhash, w.qhash_size);

@@ -197,17 +197,18 @@ __device__ void connect_one(const Ws &w, int i, int n) {
       walk.start(tree, 1, w.bucket_log, lo, hi);
       bool safe = true;
       while (true) {
-        int leaf_node = 0;
-        int c = walk.next(tree, lo, hi, leaf_node);
-        if (c < 0) break;
+        int leaf_node = walk.next_node(tree, lo, hi);
+        if (leaf_node < 0) break;
         visits += 1;
-        double2 u = w.cseg_a[c], v = w.cseg_b[c];
+        const int c = leaf_node - tree.leaves;  // the segment's record in leaf order
+        double2 u = w.cl_a[c], v = w.cl_b[c];
         double2 d = make_double2(v.x - u.x, v.y - u.y);
         double length = dlength(d);
         if (length > 0.0) {
           double t = ((x.x - u.x) * d.x + (x.y - u.y) * d.y) / (d.x * d.x + d.y * d.y);
           if (fabs(cross3(u, v, x)) <= EPS * length && t >= -EPS / length && t <= 1.0 + EPS / length) {
-            double n0 = w.cnear[c], n1 = w.cnear[w.nseg + c];
+            double2 nr = w.cl_near[c];
+            double n0 = nr.x, n1 = nr.y;
             double depth = n0 + t * (n1 - n0);
             if (depth <= z + pad) {
               safe = false;

@@ -112,8 +112,14 @@ enum Control : int {
   C_RAYSORT = 145,   // leaders to put in Morton order (0 when a wave walks each ray)
   C_HUGE_SYNC = 160, // arrivals at the grid barriers of the huge-group sort
   C_RAYKEY_DONE = 192, // blocks of k_ray_keys that have finished
+  C_QUEUE = 224,     // next free subdivision queue entry
   C_COUNT = 256,
 };
+
+// Per-wave partial counters of the subdivision kernel (visits, contour tests,
+// splits, the four binary32 filter counts, rounds), summed by the next kernel.
+constexpr int SUB_COUNTERS = 8;
+constexpr int SUB_PARTIAL_WAVES = 8192;
 
 struct __align__(16) Box {
   float4 lo, hi;
@@ -143,6 +149,15 @@ __device__ __forceinline__ int load_i(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void acquire_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
+// Release at agent scope followed by an explicit wait for every outstanding
+// memory operation, the L2 writeback of the release included. The compiler
+// drops its own wait after the writeback when no store is pending at that
+// point (an earlier wait has drained them), and a flag stored next could then
+// reach a consumer on another XCD before the data it publishes.
+__device__ __forceinline__ void release_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_s_waitcnt(0);
+}
 __device__ __forceinline__ int load_acquire(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -345,6 +360,17 @@ struct TrailWalk {
   __device__ __forceinline__ uint32_t bucket_mask(const Tree &t, int b, float4 lo, float4 hi) const {
     return blog >= 2 ? leaves_mask(t, b, lo, hi) : small_mask(t, b, lo, hi);
   }
+  // Overlap mask of `count` consecutive boxes, read four at a time (the node
+  // array has four boxes of slack past its end for groups of one or two).
+  __device__ __forceinline__ uint32_t group_mask(const Box *g, int count, float4 lo, float4 hi) const {
+    uint32_t m = 0;
+    for (int j = 0; j < count; j += 4) {
+      Box g0 = g[j], g1 = g[j + 1], g2 = g[j + 2], g3 = g[j + 3];
+      m |= ((uint32_t)overlaps(g0, lo, hi) | (uint32_t)overlaps(g1, lo, hi) << 1 |
+            (uint32_t)overlaps(g2, lo, hi) << 2 | (uint32_t)overlaps(g3, lo, hi) << 3) << j;
+    }
+    return count < 4 ? m & ((1u << count) - 1) : m;
+  }
   // every leaf of the subtree at r
   __device__ void start(const Tree &t, int r, int want, float4 lo, float4 hi) {
     plan(t, r, want);
@@ -373,14 +399,13 @@ struct TrailWalk {
     int from = first - (b << blog);
     bucket = m & (0xFFFFFFFFu << from);
   }
-  // next overlapping leaf: its id, and its heap node in `leaf_node`; -1 when done
-  __device__ int next(const Tree &t, float4 lo, float4 hi, int &leaf_node) {
+  // heap node of the next overlapping leaf; -1 when done
+  __device__ int next_node(const Tree &t, float4 lo, float4 hi) {
     while (true) {
       if (bucket) {
         int j = __ffs(bucket) - 1;
         bucket &= bucket - 1;
-        leaf_node = (cur << blog) + j;
-        return t.ids[leaf_node - t.leaves];
+        return (cur << blog) + j;
       }
       if (!trail) return -1;
       int lvl = (63 - __clzll((long long)trail)) >> 2;  // deepest level with a pending node
@@ -389,10 +414,14 @@ struct TrailWalk {
       int parent = lvl == 0 ? root : cur >> (2 * (kc - lvl + 1));
       cur = 4 * parent + idx;
       kc = lvl;
-      if (kc == lvls - 1)
-        bucket = bucket_mask(t, cur, lo, hi);
+      // one read for every lane of a wave, whether it expands an internal
+      // node (four grandchildren) or reaches a bucket of leaves
+      const bool bottom = kc == lvls - 1;
+      uint32_t m = group_mask(t.nodes + (bottom ? ((long)cur << blog) : 4L * cur), bottom ? 1 << blog : 4, lo, hi);
+      if (bottom)
+        bucket = m;
       else
-        trail |= (uint64_t)mask4(t, cur, lo, hi) << (4 * (kc + 1));
+        trail |= (uint64_t)m << (4 * (kc + 1));
     }
   }
 };

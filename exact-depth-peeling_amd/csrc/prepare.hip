@@ -696,7 +696,15 @@ void enqueue_prepare(const Ws &w, hipStream_t s, Timeline *t) {
   mark(t, s, "contour_segments");
   build_hierarchy(s, w.src_boxes, nullptr, w.nf, w.nf, false, st, w.sort);
   mark(t, s, "source_hierarchy");
-  build_hierarchy(s, w.seg_boxes, w.control + C_NSEG, 0, w.nseg, false, ct, w.sort);
+  LeafGather records;
+  records.a = w.cseg_a;
+  records.b = w.cseg_b;
+  records.near = w.cnear;
+  records.near_stride = w.nseg;
+  records.out_a = w.cl_a;
+  records.out_b = w.cl_b;
+  records.out_near = w.cl_near;
+  build_hierarchy(s, w.seg_boxes, w.control + C_NSEG, 0, w.nseg, false, ct, w.sort, records);
   mark(t, s, "contour_hierarchy");
 }
 

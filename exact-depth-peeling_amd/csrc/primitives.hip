@@ -415,7 +415,7 @@ __global__ void h_keys(const Box *boxes, const int *count, int count_static, int
 }
 
 __global__ void h_leaves(const Box *boxes, const int *count, int count_static, int capacity,
-                         const int *ids, const int *leaves, Box *nodes) {
+                         const int *ids, const int *leaves, Box *nodes, LeafGather g) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int size = *leaves;
   if (i >= size) return;
@@ -423,7 +423,15 @@ __global__ void h_leaves(const Box *boxes, const int *count, int count_static, i
   Box b;
   b.lo = make_float4(1e30f, 1e30f, 1e30f, 0.f);
   b.hi = make_float4(-1e30f, -1e30f, -1e30f, 0.f);
-  if (i < n) b = boxes[ids[i]];
+  if (i < n) {
+    int id = ids[i];
+    b = boxes[id];
+    if (g.out_a) {
+      g.out_a[i] = g.a[id];
+      g.out_b[i] = g.b[id];
+      g.out_near[i] = make_double2(g.near[id], g.near[g.near_stride + id]);
+    }
+  }
   nodes[size + i] = b;
 }
 
@@ -471,7 +479,7 @@ __global__ __launch_bounds__(256) void h_levels(Box *nodes, const int *leaves, i
     int next_width = top < HIER_GROUP ? top : HIER_GROUP;
     int next = group / next_width;
     int *counter = counters + stage * HIER_GROUP + next;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    release_fence();
     __syncthreads();
     if (threadIdx.x == 0) proceed = atomicAdd(counter, 1) == next_width - 1;
     __syncthreads();
@@ -485,7 +493,8 @@ __global__ __launch_bounds__(256) void h_levels(Box *nodes, const int *leaves, i
 }
 
 void build_hierarchy(hipStream_t s, const Box *boxes, const int *count, int count_static,
-                     int capacity, bool layered, Hierarchy &h, const SortScratch &scratch) {
+                     int capacity, bool layered, Hierarchy &h, const SortScratch &scratch,
+                     const LeafGather &gather) {
   int cap = capacity < h.size ? capacity : h.size;
   int grid = (cap + 255) / 256;
   hipLaunchKernelGGL(h_keys, dim3(grid < 1 ? 1 : grid), dim3(256), 0, s, boxes, count, count_static,
@@ -493,7 +502,7 @@ void build_hierarchy(hipStream_t s, const Box *boxes, const int *count, int coun
   radix_sort(s, h.keys, h.ids, h.alt_keys, h.alt_ids, count, count_static, cap, 0, layered ? 64 : 32,
              scratch);
   hipLaunchKernelGGL(h_leaves, dim3((h.size + 255) / 256), dim3(256), 0, s, boxes, count,
-                     count_static, cap, h.ids, h.leaves, h.nodes);
+                     count_static, cap, h.ids, h.leaves, h.nodes, gather);
   long blocks = h.size / HIER_GROUP;
   hipLaunchKernelGGL(h_levels, dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(256), 0, s, h.nodes, h.leaves,
                      h.level_counters);

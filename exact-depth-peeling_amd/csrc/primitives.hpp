@@ -62,8 +62,20 @@ struct Hierarchy {
 // Morton-ordered balanced hierarchy over leaf boxes[0, count) (warp_peel/
 // solver.py:34-89 semantics: 15-bit x/y Morton code of the binary32 box
 // centre, layer in the high word when layered, stable order, empty padding).
+// Optional per-leaf records gathered into leaf order while the leaves are
+// written: out_a[i], out_b[i], out_near[i] = (near[id], near[near_stride + id])
+// for the leaf's id = ids[i]. A walk that reaches leaf i then reads its record
+// at i, with no dependent load of the id first.
+struct LeafGather {
+  const double2 *a = nullptr, *b = nullptr;
+  const double *near = nullptr;
+  long near_stride = 0;
+  double2 *out_a = nullptr, *out_b = nullptr, *out_near = nullptr;
+};
+
 void build_hierarchy(hipStream_t s, const Box *boxes, const int *count, int count_static,
-                     int capacity, bool layered, Hierarchy &h, const SortScratch &scratch);
+                     int capacity, bool layered, Hierarchy &h, const SortScratch &scratch,
+                     const LeafGather &gather = LeafGather{});
 
 int pow2_at_least(int n);
 

@@ -50,8 +50,47 @@ __device__ __forceinline__ void connect_reset(const Ws &w, int i) {
   }
 }
 
+// The subdivision kernel's per-wave partial counters, summed by block 0 (256
+// threads) into the metrics.
+__device__ void sum_subdivide_partials(const Ws &w) {
+  __shared__ long long sums[SUB_COUNTERS][4];
+  const int waves = min(w.persistent_blocks * 4, SUB_PARTIAL_WAVES);
+  long long part[SUB_COUNTERS];
+  for (int k = 0; k < SUB_COUNTERS; ++k) part[k] = 0;
+  for (int v = threadIdx.x; v < waves; v += blockDim.x)
+    for (int k = 0; k < SUB_COUNTERS; ++k) {
+      long long x = w.sub_partials[(long)v * SUB_COUNTERS + k];
+      part[k] = k == SUB_COUNTERS - 1 ? max(part[k], x) : part[k] + x;
+    }
+  for (int k = 0; k < SUB_COUNTERS; ++k) {
+    long long x = part[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      long long y = __shfl_xor(x, off, 64);
+      x = k == SUB_COUNTERS - 1 ? max(x, y) : x + y;
+    }
+    if ((threadIdx.x & 63) == 0) sums[k][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int slot[SUB_COUNTERS - 1] = {M_VISITS, M_CONTOUR_TESTS, M_SPLITS, M_FILTER_STRADDLE,
+                                        M_FALLBACK_STRADDLE, M_FILTER_FRONT, M_FALLBACK_FRONT};
+    for (int k = 0; k < SUB_COUNTERS; ++k) {
+      long long x = sums[k][0];
+      for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+        x = k == SUB_COUNTERS - 1 ? max(x, sums[k][q]) : x + sums[k][q];
+      if (k < SUB_COUNTERS - 1)
+        w.metrics[slot[k]] += x;  // waves past the partials added theirs directly
+      else
+        w.metrics[M_ROUNDS] = max(w.metrics[M_ROUNDS], x);
+    }
+  }
+}
+
 // Resets of the rank stage and of the connectivity it starts with, one launch.
-__global__ void k_rank_reset(Ws w) {
+// Block 0 first sums the subdivision counters (the connectivity reset reads
+// the split count).
+__global__ __launch_bounds__(256) void k_rank_reset(Ws w) {
+  if (blockIdx.x == 0) sum_subdivide_partials(w);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int size = qhash_live(w);
   for (int j = i; j < size; j += gridDim.x * blockDim.x) w.qhash[j] = -1;
@@ -246,8 +285,8 @@ __global__ __launch_bounds__(256) void k_certify(Ws w) {
     walk.start(tree, 1, w.bucket_log, lo, hi);
   }
   for (int step = 0; step < 32 && node > 0; ++step) {
-    int leaf_node = 0;
-    if (walk.next(tree, lo, hi, leaf_node) < 0) break;
+    int leaf_node = walk.next_node(tree, lo, hi);
+    if (leaf_node < 0) break;
     node = successor(leaf_node);
     int leaf = leaf_node - tree.leaves;
     visits += 1;
@@ -294,8 +333,8 @@ __global__ __launch_bounds__(256) void k_certify_heavy(Ws w, int workers) {
     else
       walk.start(tree, root, w.bucket_log, lo, hi);
     while (true) {
-      int leaf_node = 0;
-      if (walk.next(tree, lo, hi, leaf_node) < 0) break;
+      int leaf_node = walk.next_node(tree, lo, hi);
+      if (leaf_node < 0) break;
       int leaf = leaf_node - tree.leaves;
       visits += 1;
       if (!covers_leaf(w, leaf, corner)) {
@@ -561,7 +600,7 @@ __global__ __launch_bounds__(256) void k_ray_keys(Ws w) {
     }
     __syncthreads();
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  release_fence();
   __syncthreads();
   if (threadIdx.x == 0) last = atomicAdd(w.control + C_RAYKEY_DONE, 1) == (int)gridDim.x - 1;
   __syncthreads();
@@ -623,8 +662,8 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
         }
       }
       if (live) {
-        int leaf_node = 0;
-        if (walk.next(tree, lo, hi, leaf_node) < 0) {
+        int leaf_node = walk.next_node(tree, lo, hi);
+        if (leaf_node < 0) {
           live = false;
         } else {
           node = successor(leaf_node);
@@ -710,8 +749,8 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
       bool hit = false;
       int leaf = 0;
       if (live) {
-        int leaf_node = 0;
-        if (walk.next(tree, lo, hi, leaf_node) < 0) {
+        int leaf_node = walk.next_node(tree, lo, hi);
+        if (leaf_node < 0) {
           live = false;
         } else {
           leaf = leaf_node - tree.leaves;

@@ -6,24 +6,30 @@
 // node (warp_peel/events.py:222-264, 804). HIP graphs have no conditional
 // nodes, so the rounds become a device work queue instead:
 //
-//  * Tasks are slots. Slots [0, nf) are the projected sources themselves
-//    (seeds, read in place from the source rings); every cut allocates two
-//    consecutive frontier slots [nf + k, nf + k + 2).
-//  * A lane claims the next ticket (= slot index, wave-aggregated atomicAdd)
-//    and processes it once the slot's ready flag is published (release/acquire
-//    at agent scope). Per visit it inspects at most 32 contour candidates from
-//    the slot's hierarchy cursor, exactly like one reference round; a visit
-//    without a cut keeps the task and continues from the cursor.
-//  * C_PENDING counts live tasks: seeds start it at nf, a cut adds two and
-//    retires one (a single +1), a finished task subtracts one. It only reaches
-//    zero when no task is live and no cut is in flight, so a lane that holds a
-//    ticket that is not ready and sees zero can exit: the ticket will never be
-//    allocated. Every wave reaches that exit; an error bit also exits all
-//    lanes, and a wall-clock guard turns a stall into error bit 2048.
+//  * Tickets [0, nf) are the projected sources themselves (seeds, read in
+//    place from the source rings); ticket nf + q is queue entry q. Every cut
+//    stores its two children in two consecutive frontier slots k, k + 1.
+//  * The lane that cuts goes on with the left child itself (depth first, no
+//    hand-off) and publishes the right child as the next queue entry: the
+//    entry holds 1 + the child's slot, stored after a release fence.
+//  * A lane claims the next ticket (wave-aggregated atomicAdd) and processes
+//    it once its entry is published (acquire at agent scope). Per visit it
+//    inspects at most 32 contour candidates from the task's hierarchy cursor,
+//    exactly like one reference round; a visit without a cut keeps the task
+//    and continues from the cursor.
+//  * C_PENDING counts live tasks: seeds start it at nf, a split adds one (the
+//    right child; the left child takes the parent's place), a finished task
+//    subtracts one. It only reaches zero when no task is live and no cut is
+//    in flight, so a lane that holds a ticket that is not published and sees
+//    zero can exit: the entry will never be published. Every wave reaches that
+//    exit; an error bit also exits all lanes, and a wall-clock guard turns a
+//    stall into error bit 2048.
 //
 // Each task's sequence of cuts depends only on its own ring and cursor, so the
 // set of final patches is the same as the reference's round-synchronous
-// frontier for the same hierarchy (only the output order differs).
+// frontier for the same hierarchy (only the output order differs). A cut
+// whose child collapses carries the parent on (frontier.py:200-206): here the
+// lane simply goes on with the same ring from the new cursor.
 #include "workspace.hpp"
 
 namespace peel {
@@ -32,7 +38,7 @@ namespace peel {
 // set by the pipeline's first kernel, k_reset.
 
 struct Slot {
-  const double2 *xy;  // vertex j at xy[j * stride]
+  double2 *xy;  // vertex j at xy[j * stride]
   int stride;
 };
 
@@ -44,28 +50,40 @@ __device__ __forceinline__ Slot slot_of(const Ws &w, int task) {
 __device__ __forceinline__ double2 vtx(const Slot &s, int j) { return s.xy[(long)j * s.stride]; }
 
 
-// The first SUB_LDS vertices of a lane's current ring live in LDS,
+// Each lane has three rings in LDS: its current ring and the two children of
+// a cut, which are built and cleaned there (a cut and its cleanup make no
+// global round trips; only the right child, which another lane takes, is
+// copied to its slot). The first SUB_LDS vertices of a ring live in LDS,
 // vertex-major so the 64 lanes of a wave read vertex j of their rings from
-// consecutive banks; the rare longer ring reads the rest from its slot (slots
-// are never reused within a run). 192 B of LDS per lane lets several blocks
-// share a CU.
+// consecutive banks; the rare longer ring keeps the rest in its slot (slots
+// are never reused within a run). 576 B of LDS per lane: one block per CU.
 constexpr int SUB_THREADS = 256;
 constexpr int SUB_LDS = 12;
-__shared__ double2 sub_ring_lds[SUB_LDS * SUB_THREADS];
-// kept out of line so the compiler does not fold the LDS and global loads
-// into one generic load
+constexpr int SUB_RING = SUB_LDS * SUB_THREADS;  // LDS words of one ring for the block
+__shared__ double2 sub_ring_lds[3 * SUB_RING];
+// kept out of line so the compiler does not fold the LDS and global accesses
+// into generic ones
 __device__ __noinline__ double2 slot_vertex(const double2 *slot, long stride, int j) {
   return slot[(long)j * stride];
 }
+__device__ __noinline__ void slot_store(double2 *slot, long stride, int j, double2 v) {
+  slot[(long)j * stride] = v;
+}
 struct LdsRing {
-  int lane;             // this lane's column of sub_ring_lds
-  const double2 *slot;  // the task's slot: vertex j at slot[j * stride]
+  int lane;       // this lane's column
+  int base;       // which of the lane's three rings: 0, SUB_RING or 2 * SUB_RING
+  double2 *slot;  // the ring's slot: vertex j at slot[j * stride]
   long stride;
   __device__ double2 operator[](int j) const {
-    if (j < SUB_LDS) return sub_ring_lds[j * SUB_THREADS + lane];
+    if (j < SUB_LDS) return sub_ring_lds[base + j * SUB_THREADS + lane];
     return slot_vertex(slot, stride, j);
   }
-  __device__ void put(int j, double2 v) const { sub_ring_lds[j * SUB_THREADS + lane] = v; }
+  __device__ void put(int j, double2 v) const {
+    if (j < SUB_LDS)
+      sub_ring_lds[base + j * SUB_THREADS + lane] = v;
+    else
+      slot_store(slot, stride, j, v);
+  }
 };
 
 // frontier.py:81-103
@@ -123,9 +141,39 @@ __device__ bool front_segment32(const LdsRing &s, int n, double2 a, double2 b, d
   return high > low;
 }
 
-__device__ __forceinline__ int append_slot(double2 *xy, int stride, int n, double2 x) {
-  if (n < MAXV) xy[(long)n * stride] = x;
+__device__ __forceinline__ int append(const LdsRing &r, int n, double2 x) {
+  if (n < MAXV) r.put(n, x);
   return n + 1;
+}
+
+// clean (frontier.py:45-71 = kernels.py:146-176), in place on a lane's ring;
+// the same operations as clean_slot, in the same order.
+__device__ int clean_ring(const LdsRing &r, int n) {
+  if (n > MAXV) return n;
+  int count = 0;
+  double2 last = make_double2(0, 0);
+  for (int j = 0; j < n; ++j) {
+    double2 x = r[j];
+    if (count == 0 || dlength(make_double2(x.x - last.x, x.y - last.y)) > EPS * 0.01) {
+      r.put(count, x);
+      last = x;
+      count += 1;
+    }
+  }
+  if (count > 1) {
+    double2 first = r[0];
+    if (dlength(make_double2(first.x - last.x, first.y - last.y)) <= EPS * 0.01) count -= 1;
+  }
+  if (count < 3) return 0;
+  double2 origin = r[0];
+  double area2 = 0.0, perimeter = 0.0;
+  for (int j = 0; j < count; ++j) {
+    double2 a = r[j], b = r[(j + 1) % count];
+    area2 += cross3(origin, a, b);
+    perimeter += fabs(b.x - a.x) + fabs(b.y - a.y);
+  }
+  if (area2 <= EPS * perimeter) return 0;
+  return count;
 }
 
 // save_polygon (kernels.py:579-596) with layer 0; ranks are assigned later.
@@ -142,9 +190,13 @@ __device__ void save_output(const Ws &w, const LdsRing &s, int n, int source, in
 }
 
 __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
-  LdsRing ring{(int)threadIdx.x, nullptr, 0};
+  LdsRing ring{(int)threadIdx.x, 0, nullptr, 0};
   TrailWalk walk;
   bool fresh = false;  // the task's walk starts at its first visit, from its binary cursor
+  // A child that this lane goes on with may keep its parent's walk: the
+  // parent's box (plo, phi) bounds the masks the walk holds.
+  bool inherit = false;
+  float4 plo = make_float4(0.f, 0.f, 0.f, 0.f), phi = plo;
   const int nf = w.nf;
   const Tree tree = w.contour_tree.tree();
   int lane = threadIdx.x & 63;
@@ -167,14 +219,14 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
         base = __shfl(base, leader);
         if (need) ticket = base + __popcll(mask & lt);
       }
-      // A ticket past the frontier capacity can never be allocated: it only
+      // A ticket past the frontier capacity can never be published: it only
       // waits for the pending count to reach zero. Polling is relaxed; the
-      // acquire fence runs once, when a published slot is taken.
-      bool ready = ticket < nf || (ticket - nf < w.cap.frontier &&
-                                   load_i(w.fr_ready + (ticket - nf)) != 0);
+      // acquire fence runs once, when a published entry is taken.
+      int entry = ticket >= nf && ticket - nf < w.cap.frontier ? load_i(w.fr_ready + (ticket - nf)) : 0;
+      bool ready = ticket < nf || entry != 0;
       if (__ballot(ready && ticket >= nf)) acquire_fence();
       if (ready) {
-        task = ticket;
+        task = ticket < nf ? ticket : nf + entry - 1;
         ticket = -1;
         if (task < nf) {
           n = w.src_n[task];
@@ -235,25 +287,37 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
       // the leaves still to visit are those from the leftmost leaf under the
       // cursor on (a task continues after the candidate that cut its parent)
       fresh = false;
-      walk.trail = 0;
-      walk.bucket = 0;
-      if (node > 0) {
-        int first = node << (__clz(node) - __clz(tree.leaves));
-        walk.start_at(tree, 1, first, w.bucket_log, lo, hi);
+      if (inherit && lo.x >= plo.x && lo.y >= plo.y && hi.x <= phi.x && hi.y <= phi.y && hi.z <= phi.z) {
+        // Inside the parent's box: every node the child's walk would visit
+        // overlapped the parent's box, so it is pending in the parent's walk.
+        // Pending nodes are tested against the child's box when expanded;
+        // only the current bucket's pending leaves need testing again.
+        if (walk.bucket) walk.bucket &= walk.bucket_mask(tree, walk.cur, lo, hi);
+      } else {
+        walk.trail = 0;
+        walk.bucket = 0;
+        if (node == 1) {
+          walk.start(tree, 1, w.bucket_log, lo, hi);  // a seed: from the root
+        } else if (node > 0) {
+          int first = node << (__clz(node) - __clz(tree.leaves));
+          walk.start_at(tree, 1, first, w.bucket_log, lo, hi);
+        }
       }
+      inherit = false;
     }
     bool done = false, cut = false;
     for (int step = 0; step < 32; ++step) {
-      int leaf = 0;
-      int c = walk.next(tree, lo, hi, leaf);
-      if (c < 0) {
+      int leaf = walk.next_node(tree, lo, hi);
+      if (leaf < 0) {
         node = 0;
         done = true;
         break;
       }
       node = successor(leaf);
       tests += 1;
-      double2 a = w.cseg_a[c], b = w.cseg_b[c];
+      // the candidate's record in leaf order: endpoints and minimum depths
+      const int c = leaf - tree.leaves;
+      double2 a = w.cl_a[c], b = w.cl_b[c], near = w.cl_near[c];
       bool positive = false, negative = false;
       float tol = 0.f;
       if (w.filter32) {
@@ -285,8 +349,8 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
         }
       }
       if (!(positive && negative)) continue;
-      double za = value(score, a) - w.cnear[c];
-      double zb = value(score, b) - w.cnear[w.nseg + c];
+      double za = value(score, a) - near.x;
+      double zb = value(score, b) - near.y;
       if (w.filter32 ? !front_segment32(s, n, a, b, za, zb, tol, f_front, b_front)
                      : !front_segment(s, n, a, b, za, zb))
         continue;
@@ -297,25 +361,28 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
         cut = true;
         break;
       }
-      int fcap = w.cap.frontier;
-      double2 *L = w.fr_xy + k, *R = w.fr_xy + k + 1;
+      const int fcap = w.cap.frontier;
+      // the children in the lane's two other LDS rings (slots k and k + 1
+      // hold vertices past SUB_LDS)
+      const LdsRing L{ring.lane, (ring.base + SUB_RING) % (3 * SUB_RING), w.fr_xy + k, fcap};
+      const LdsRing R{ring.lane, (ring.base + 2 * SUB_RING) % (3 * SUB_RING), w.fr_xy + k + 1, fcap};
       int left = 0, right = 0;
       double2 d = make_double2(b.x - a.x, b.y - a.y);
       for (int j = 0; j < n; ++j) {
         double2 u = s[j], v = s[(j + 1) % n];
         double du = -d.y * (u.x - a.x) + d.x * (u.y - a.y) + 0.0;
         double dv = -d.y * (v.x - a.x) + d.x * (v.y - a.y) + 0.0;
-        if (du >= 0.0) left = append_slot(L, fcap, left, u);
-        if (du <= 0.0) right = append_slot(R, fcap, right, u);
+        if (du >= 0.0) left = append(L, left, u);
+        if (du <= 0.0) right = append(R, right, u);
         if ((du > 0.0 && dv < 0.0) || (du < 0.0 && dv > 0.0)) {
           double t = du / (du - dv);
           double2 x = make_double2(u.x + t * (v.x - u.x), u.y + t * (v.y - u.y));
-          left = append_slot(L, fcap, left, x);
-          right = append_slot(R, fcap, right, x);
+          left = append(L, left, x);
+          right = append(R, right, x);
         }
       }
-      left = clean_slot(L, fcap, left);
-      right = clean_slot(R, fcap, right);
+      left = clean_ring(L, left);
+      right = clean_ring(R, right);
       if (left > MAXV || right > MAXV) {
         set_error(w.status, ERR_POLY);
         cut = true;
@@ -323,54 +390,58 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
       }
       if (left > 0 && right > 0) {
         splits += 1;
-      } else {
-        // A collapsed child cancels the cut: carry the parent (frontier.py:200-206).
-        for (int j = 0; j < n; ++j) L[(long)j * fcap] = s[j];
-        left = n;
-        right = 0;
+        // the right child goes to the queue, whole in its slot
+        for (int j = 0; j < right && j < SUB_LDS; ++j) R.slot[(long)j * fcap] = R[j];
+        w.fr_n[k + 1] = right;
+        w.fr_src[k + 1] = source;
+        w.fr_node[k + 1] = node;
+        w.fr_gen[k + 1] = gen + 1;
+        int q = atomicAdd(w.control + C_QUEUE, 1);
+        release_fence();
+        __hip_atomic_store(w.fr_ready + q, k + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int live = atomicAdd(w.control + C_PENDING, 1) + 1;
+        atomicMax(w.status + ST_PEAK, live);
+        // and this lane goes on with the left child
+        ring = L;
+        n = left;
       }
-      w.fr_n[k] = left;
-      w.fr_n[k + 1] = right;
-      w.fr_src[k] = source;
-      w.fr_src[k + 1] = source;
-      w.fr_node[k] = node;
-      w.fr_node[k + 1] = node;
-      w.fr_gen[k] = gen + 1;
-      w.fr_gen[k + 1] = gen + 1;
-      // one release for both children, then the two ready flags
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      __hip_atomic_store(w.fr_ready + k, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(w.fr_ready + k + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int live = atomicAdd(w.control + C_PENDING, 1) + 1;
-      atomicMax(w.status + ST_PEAK, live);
+      // a collapsed child leaves the parent's ring to go on from the cursor
+      inherit = true;
+      plo = lo;
+      phi = hi;
       cut = true;
       break;
     }
     bool finish = !cut && done;
     int o = wave_claim32(w.status + ST_POLYS, finish);
-    if (cut) {
-      task = -1;
-    } else if (finish) {
+    if (finish) {
       save_output(w, s, n, source, o);
       atomicSub(w.control + C_PENDING, 1);
       task = -1;
     } else {
-      gen += 1;  // cursor continues in the next visit, like a re-enqueued patch
+      // a child (or a task whose visit ran out of candidates) continues from
+      // the cursor in the next visit, like a re-enqueued patch
+      gen += 1;
+      fresh = fresh || cut;
     }
   }
-  // one atomic per wave for the counters
-  wave_add_metric(&w.metrics[M_VISITS], visits);
-  wave_add_metric(&w.metrics[M_CONTOUR_TESTS], tests);
-  wave_add_metric(&w.metrics[M_SPLITS], splits);
-  if (w.filter32) {
-    wave_add_metric(&w.metrics[M_FILTER_STRADDLE], f_straddle);
-    wave_add_metric(&w.metrics[M_FALLBACK_STRADDLE], b_straddle);
-    wave_add_metric(&w.metrics[M_FILTER_FRONT], f_front);
-    wave_add_metric(&w.metrics[M_FALLBACK_FRONT], b_front);
+  // Per-wave partial counters, summed by the next kernel: same-address
+  // atomics from every wave at once would serialise at the queue's end.
+  long long part[SUB_COUNTERS] = {visits, tests, splits, f_straddle, b_straddle, f_front, b_front, 0};
+  for (int k = 0; k < SUB_COUNTERS - 1; ++k) part[k] = wave_sum(part[k]);
+  part[SUB_COUNTERS - 1] = wave_max(max_gen) + 1;
+  const int wave_id = blockIdx.x * (SUB_THREADS / 64) + (threadIdx.x >> 6);
+  if ((threadIdx.x & 63) == 0) {
+    if (wave_id < SUB_PARTIAL_WAVES) {
+      for (int k = 0; k < SUB_COUNTERS; ++k) w.sub_partials[(long)wave_id * SUB_COUNTERS + k] = part[k];
+    } else {
+      const int slot[SUB_COUNTERS - 1] = {M_VISITS, M_CONTOUR_TESTS, M_SPLITS, M_FILTER_STRADDLE,
+                                          M_FALLBACK_STRADDLE, M_FILTER_FRONT, M_FALLBACK_FRONT};
+      for (int k = 0; k < SUB_COUNTERS - 1; ++k)
+        if (part[k]) atomicAdd((unsigned long long *)&w.metrics[slot[k]], (unsigned long long)part[k]);
+      atomicMax((unsigned long long *)&w.metrics[M_ROUNDS], (unsigned long long)part[SUB_COUNTERS - 1]);
+    }
   }
-  max_gen = wave_max(max_gen);
-  if ((threadIdx.x & 63) == 0)
-    atomicMax((unsigned long long *)&w.metrics[M_ROUNDS], (unsigned long long)(max_gen + 1));
 }
 
 void enqueue_subdivide(const Ws &w, hipStream_t s, Timeline *t) {

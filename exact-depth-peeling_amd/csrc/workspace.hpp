@@ -70,6 +70,10 @@ struct Ws {
   int *seg_flag, *seg_off; // [nseg] unique-segment flags and their scan
   double2 *cseg_a, *cseg_b;  // [nseg] compacted unique segments
   double *cnear;           // [2][nseg] their minimum endpoint depths
+  // the unique segments again in contour-hierarchy leaf order (endpoints and
+  // the two minimum depths), in the memory of seg_a, seg_b and seg_near,
+  // which are dead once the segments are compacted
+  double2 *cl_a, *cl_b, *cl_near;
 
   // hierarchies
   Hierarchy source_tree, contour_tree;
@@ -83,7 +87,9 @@ struct Ws {
 
   // frontier pool [MAXV][frontier]
   double2 *fr_xy;
-  int *fr_n, *fr_src, *fr_node, *fr_gen, *fr_ready;
+  int *fr_n, *fr_src, *fr_node, *fr_gen;
+  int *fr_ready;  // queue entry q: 0 until published, then 1 + the task's frontier slot
+  long long *sub_partials;  // [SUB_PARTIAL_WAVES][SUB_COUNTERS]
 
   // output polygons [MAXV][regions]
   double2 *out_xy;
