@@ -252,9 +252,11 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
         }
       }
     }
+    // (a ballot inside the branch below would only see the idle lanes)
+    const bool wave_has_task = __ballot(task >= 0) != 0;
     if (task < 0) {
       if (load_i(w.control + C_PENDING) == 0) break;
-      if (__ballot(task >= 0) == 0) {
+      if (!wave_has_task) {
         __builtin_amdgcn_s_sleep(32);
         if (wall_clock64() - idle_since > 1000000000ull) {
           set_error(w.status, ERR_ROUNDS);
