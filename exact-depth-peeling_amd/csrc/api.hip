@@ -204,7 +204,6 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
   c.take(w.fr_src, F);
   c.take(w.fr_node, F);
   c.take(w.fr_gen, F);
-  c.take(w.fr_ready, F);
   c.take(w.sub_partials, layered ? 1L : (long)SUB_PARTIAL_WAVES * SUB_COUNTERS);
   c.take(w.samples, R);
   c.take(w.reps, R);

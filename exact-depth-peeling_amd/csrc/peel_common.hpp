@@ -92,8 +92,7 @@ enum Metric : int {
 // Contended words sit on their own 128-byte lines.
 enum Control : int {
   C_ALLOC = 0,      // next free frontier slot
-  C_HEAD = 32,      // next ticket
-  C_PENDING = 64,   // live frontier tasks
+  C_HEAD = 32,      // next seed of the subdivision
   C_HEAVY = 96,     // heavy ray jobs
   C_NBV = 129,      // boundary vertices (scan total)
   C_MEDIUM = 131,   // groups sorted by a workgroup
@@ -112,13 +111,14 @@ enum Control : int {
   C_RAYSORT = 145,   // leaders to put in Morton order (0 when a wave walks each ray)
   C_HUGE_SYNC = 160, // arrivals at the grid barriers of the huge-group sort
   C_RAYKEY_DONE = 192, // blocks of k_ray_keys that have finished
-  C_QUEUE = 224,     // next free subdivision queue entry
   C_COUNT = 256,
 };
 
-// Per-wave partial counters of the subdivision kernel (visits, contour tests,
-// splits, the four binary32 filter counts, rounds), summed by the next kernel.
-constexpr int SUB_COUNTERS = 8;
+// Per-wave partial counters of the subdivision kernel, reduced by the next
+// kernel: sums (visits, contour tests, splits, the four binary32 filter
+// counts), then maxima (rounds, the peak of a block's live tasks).
+constexpr int SUB_SUMS = 7;
+constexpr int SUB_COUNTERS = SUB_SUMS + 2;
 constexpr int SUB_PARTIAL_WAVES = 8192;
 
 struct __align__(16) Box {

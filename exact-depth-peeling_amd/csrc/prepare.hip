@@ -21,10 +21,7 @@ __global__ void k_reset(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 16) w.status[i] = 0;
   if (i < M_COUNT) w.metrics[i] = 0;
-  // the subdivision's work queue starts with every source as a live task
-  if (i < C_COUNT) w.control[i] = i == C_PENDING ? w.nf : 0;
-  if (w.route != ROUTE_LAYERED)
-    for (int j = i; j < w.cap.frontier; j += gridDim.x * blockDim.x) w.fr_ready[j] = 0;
+  if (i < C_COUNT) w.control[i] = 0;
   for (int j = i; j < w.vhash_size; j += gridDim.x * blockDim.x) w.vhash[j] = -1;
   for (int j = i; j < w.ehash_size; j += gridDim.x * blockDim.x) w.etable[j] = EdgeSlot{~0ull, 0ull};
   for (int j = i; j < w.shash_size; j += gridDim.x * blockDim.x) w.shash[j] = -1;

@@ -60,28 +60,30 @@ __device__ void sum_subdivide_partials(const Ws &w) {
   for (int v = threadIdx.x; v < waves; v += blockDim.x)
     for (int k = 0; k < SUB_COUNTERS; ++k) {
       long long x = w.sub_partials[(long)v * SUB_COUNTERS + k];
-      part[k] = k == SUB_COUNTERS - 1 ? max(part[k], x) : part[k] + x;
+      part[k] = k >= SUB_SUMS ? max(part[k], x) : part[k] + x;
     }
   for (int k = 0; k < SUB_COUNTERS; ++k) {
     long long x = part[k];
     for (int off = 32; off > 0; off >>= 1) {
       long long y = __shfl_xor(x, off, 64);
-      x = k == SUB_COUNTERS - 1 ? max(x, y) : x + y;
+      x = k >= SUB_SUMS ? max(x, y) : x + y;
     }
     if ((threadIdx.x & 63) == 0) sums[k][threadIdx.x >> 6] = x;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int slot[SUB_COUNTERS - 1] = {M_VISITS, M_CONTOUR_TESTS, M_SPLITS, M_FILTER_STRADDLE,
-                                        M_FALLBACK_STRADDLE, M_FILTER_FRONT, M_FALLBACK_FRONT};
+    const int slot[SUB_SUMS] = {M_VISITS, M_CONTOUR_TESTS, M_SPLITS, M_FILTER_STRADDLE,
+                                M_FALLBACK_STRADDLE, M_FILTER_FRONT, M_FALLBACK_FRONT};
     for (int k = 0; k < SUB_COUNTERS; ++k) {
       long long x = sums[k][0];
       for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
-        x = k == SUB_COUNTERS - 1 ? max(x, sums[k][q]) : x + sums[k][q];
-      if (k < SUB_COUNTERS - 1)
+        x = k >= SUB_SUMS ? max(x, sums[k][q]) : x + sums[k][q];
+      if (k < SUB_SUMS)
         w.metrics[slot[k]] += x;  // waves past the partials added theirs directly
-      else
+      else if (k == SUB_SUMS)
         w.metrics[M_ROUNDS] = max(w.metrics[M_ROUNDS], x);
+      else
+        w.status[ST_PEAK] = max(w.status[ST_PEAK], (int)x);
     }
   }
 }

@@ -6,24 +6,25 @@
 // node (warp_peel/events.py:222-264, 804). HIP graphs have no conditional
 // nodes, so the rounds become a device work queue instead:
 //
-//  * Tickets [0, nf) are the projected sources themselves (seeds, read in
-//    place from the source rings); ticket nf + q is queue entry q. Every cut
-//    stores its two children in two consecutive frontier slots k, k + 1.
+//  * Seeds (the projected sources, read in place from their rings) are
+//    handed out by one global ticket counter, one wave-aggregated claim at a
+//    time. Every cut stores its two children in two consecutive frontier
+//    slots k, k + 1.
 //  * The lane that cuts goes on with the left child itself (depth first, no
-//    hand-off) and publishes the right child as the next queue entry: the
-//    entry holds 1 + the child's slot, stored after a release fence.
-//  * A lane claims the next ticket (wave-aggregated atomicAdd) and processes
-//    it once its entry is published (acquire at agent scope). Per visit it
-//    inspects at most 32 contour candidates from the task's hierarchy cursor,
-//    exactly like one reference round; a visit without a cut keeps the task
-//    and continues from the cursor.
-//  * C_PENDING counts live tasks: seeds start it at nf, a split adds one (the
-//    right child; the left child takes the parent's place), a finished task
-//    subtracts one. It only reaches zero when no task is live and no cut is
-//    in flight, so a lane that holds a ticket that is not published and sees
-//    zero can exit: the entry will never be published. Every wave reaches that
-//    exit; an error bit also exits all lanes, and a wall-clock guard turns a
-//    stall into error bit 2048.
+//    hand-off); the right child goes to its block's queue in LDS and is
+//    taken by an idle lane of the same block. Children never leave their
+//    block, so handing one over needs no agent-scope fence: the child's data
+//    is stored by the same CU that reads it (workgroup scope), and agent-scope
+//    fences would slow every wave's loads (each acquire invalidates caches,
+//    each release writes the L2 back).
+//  * Per visit a lane inspects at most 32 contour candidates from the task's
+//    hierarchy cursor, exactly like one reference round; a visit without a
+//    cut keeps the task and continues from the cursor.
+//  * Each block counts its live tasks in LDS: a seed taken adds one, a queued
+//    right child adds one, a finished task subtracts one. A lane leaves once
+//    the seeds are all handed out and its block has no live task. An error
+//    bit also exits all lanes, and a wall-clock guard turns a stall into
+//    error bit 2048.
 //
 // Each task's sequence of cuts depends only on its own ring and cursor, so the
 // set of final patches is the same as the reference's round-synchronous
@@ -34,8 +35,8 @@
 
 namespace peel {
 
-// The queue's ready flags and counters (C_ALLOC, C_HEAD, C_PENDING = nf) are
-// set by the pipeline's first kernel, k_reset.
+// The seed counter and the frontier slot counter (C_HEAD, C_ALLOC) are
+// cleared by the pipeline's first kernel, k_reset.
 
 struct Slot {
   double2 *xy;  // vertex j at xy[j * stride]
@@ -61,6 +62,14 @@ constexpr int SUB_THREADS = 256;
 constexpr int SUB_LDS = 12;
 constexpr int SUB_RING = SUB_LDS * SUB_THREADS;  // LDS words of one ring for the block
 __shared__ double2 sub_ring_lds[3 * SUB_RING];
+// The block's queue of right children (frontier slots; -1: empty), a ring of
+// BQ entries with monotonic head and tail, and the block's live tasks.
+constexpr int BQ = 2048;
+constexpr int BQ_MARGIN = SUB_THREADS;  // room for every lane of the block to push at once
+__shared__ int sub_queue[BQ];
+__shared__ int sub_head, sub_tail, sub_live;
+// a lane keeps up to DEFER right children itself when its block's queue is full
+constexpr int DEFER = 4;
 // kept out of line so the compiler does not fold the LDS and global accesses
 // into generic ones
 __device__ __noinline__ double2 slot_vertex(const double2 *slot, long stride, int j) {
@@ -201,63 +210,109 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
   const Tree tree = w.contour_tree.tree();
   int lane = threadIdx.x & 63;
   uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  int task = -1, ticket = -1;
+  int task = -1;
   int n = 0, source = 0, node = 1, gen = 0;
   long long visits = 0, tests = 0, splits = 0;
   int f_straddle = 0, b_straddle = 0, f_front = 0, b_front = 0;
-  int max_gen = 0;
+  int max_gen = 0, peak = 0;
+  int deferred[DEFER], ndefer = 0;  // right children this lane keeps (queue full)
+  bool seeds_left = true;           // wave-uniform
   uint64_t idle_since = wall_clock64();
-  while (true) {
-    if (load_i(w.status + ST_ERR) != 0) break;
-    if (task < 0) {
-      bool need = ticket < 0;
-      uint64_t mask = __ballot(need);
+  for (int i = threadIdx.x; i < BQ; i += SUB_THREADS) sub_queue[i] = -1;
+  if (threadIdx.x == 0) sub_head = sub_tail = sub_live = 0;
+  __syncthreads();
+  for (int iter = 0;; ++iter) {
+    if ((iter & 7) == 0 && load_i(w.status + ST_ERR) != 0) break;
+    // ---- a lane without a task takes one of its own deferred children, else
+    // one from its block's queue, else a seed
+    int got = -1;  // frontier slot of a child taken
+    bool took = false;
+    if (task < 0 && ndefer > 0) {
+      ndefer -= 1;
+#pragma unroll
+      for (int d = 0; d < DEFER; ++d)
+        if (d == ndefer) got = deferred[d];
+    }
+    {
+      const bool need = task < 0 && got < 0;
+      const uint64_t mask = __ballot(need);
       if (mask) {
-        int leader = __ffsll((unsigned long long)mask) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(w.control + C_HEAD, __popcll(mask));
-        base = __shfl(base, leader);
-        if (need) ticket = base + __popcll(mask & lt);
-      }
-      // A ticket past the frontier capacity can never be published: it only
-      // waits for the pending count to reach zero. Polling is relaxed; the
-      // acquire fence runs once, when a published entry is taken.
-      int entry = ticket >= nf && ticket - nf < w.cap.frontier ? load_i(w.fr_ready + (ticket - nf)) : 0;
-      bool ready = ticket < nf || entry != 0;
-      if (__ballot(ready && ticket >= nf)) acquire_fence();
-      if (ready) {
-        task = ticket < nf ? ticket : nf + entry - 1;
-        ticket = -1;
-        if (task < nf) {
-          n = w.src_n[task];
-          source = task;
-          node = 1;
-          gen = 0;
-        } else {
-          int q = task - nf;
-          n = w.fr_n[q];
-          source = w.fr_src[q];
-          node = w.fr_node[q];
-          gen = w.fr_gen[q];
+        const int count = __popcll((unsigned long long)mask), rank = __popcll(mask & lt);
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        int first = 0, take = 0;
+        if (lane == leader) {
+          int h = __hip_atomic_load(&sub_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          while (true) {
+            int t = __hip_atomic_load(&sub_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            take = min(count, t - h);
+            if (take <= 0) {
+              take = 0;
+              break;
+            }
+            int old = atomicCAS(&sub_head, h, h + take);
+            if (old == h) break;
+            h = old;
+          }
+          first = h;
         }
-        if (n == 0) {
-          atomicSub(w.control + C_PENDING, 1);
-          task = -1;
-        } else {
-          Slot src = slot_of(w, task);
-          for (int j = 0; j < n && j < SUB_LDS; ++j) ring.put(j, vtx(src, j));
-          ring.slot = src.xy;
-          ring.stride = src.stride;
-          fresh = true;
+        first = __shfl(first, leader);
+        take = __shfl(take, leader);
+        if (need && rank < take) {
+          // the producer stores the entry right after reserving it
+          const int pos = (first + rank) & (BQ - 1);
+          int slot;
+          while ((slot = __hip_atomic_load(&sub_queue[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) < 0) {
+          }
+          __hip_atomic_store(&sub_queue[pos], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          got = slot;
+        }
+        const bool want = need && rank >= take && seeds_left;
+        const uint64_t smask = __ballot(want);
+        if (smask) {
+          const int sl = __ffsll((unsigned long long)smask) - 1, scount = __popcll((unsigned long long)smask);
+          int base = 0;
+          if (lane == sl) base = atomicAdd(w.control + C_HEAD, scount);
+          base = __shfl(base, sl);
+          if (base + scount >= nf) seeds_left = false;
+          const int ticket = base + __popcll(smask & lt);
+          if (want && ticket < nf) {
+            n = w.src_n[ticket];
+            if (n > 0) {
+              task = ticket;
+              source = ticket;
+              node = 1;
+              gen = 0;
+              took = true;
+              atomicAdd(&sub_live, 1);
+            }
+          }
         }
       }
+    }
+    if (got >= 0) {
+      // a right child of this block: its slot was stored by this CU
+      task = nf + got;
+      n = w.fr_n[got];
+      source = w.fr_src[got];
+      node = w.fr_node[got];
+      gen = w.fr_gen[got];
+      took = true;
+    }
+    if (took) {
+      Slot src = slot_of(w, task);
+      for (int j = 0; j < n && j < SUB_LDS; ++j) ring.put(j, vtx(src, j));
+      ring.slot = src.xy;
+      ring.stride = src.stride;
+      fresh = true;
     }
     // (a ballot inside the branch below would only see the idle lanes)
     const bool wave_has_task = __ballot(task >= 0) != 0;
     if (task < 0) {
-      if (load_i(w.control + C_PENDING) == 0) break;
+      if (!seeds_left && ndefer == 0 &&
+          __hip_atomic_load(&sub_live, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+        break;
       if (!wave_has_task) {
-        __builtin_amdgcn_s_sleep(32);
+        __builtin_amdgcn_s_sleep(8);
         if (wall_clock64() - idle_since > 1000000000ull) {
           set_error(w.status, ERR_ROUNDS);
           break;
@@ -398,11 +453,24 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
         w.fr_src[k + 1] = source;
         w.fr_node[k + 1] = node;
         w.fr_gen[k + 1] = gen + 1;
-        int q = atomicAdd(w.control + C_QUEUE, 1);
-        release_fence();
-        __hip_atomic_store(w.fr_ready + q, k + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int live = atomicAdd(w.control + C_PENDING, 1) + 1;
-        atomicMax(w.status + ST_PEAK, live);
+        // counted live before it can be taken; the slot's stores complete
+        // before the queue entry that names it
+        int live = atomicAdd(&sub_live, 1) + 1;
+        peak = max(peak, live);
+        __builtin_amdgcn_s_waitcnt(0);
+        int tail = __hip_atomic_load(&sub_tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        int head = __hip_atomic_load(&sub_head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (tail - head < BQ - BQ_MARGIN) {
+          int pos = atomicAdd(&sub_tail, 1) & (BQ - 1);
+          __hip_atomic_store(&sub_queue[pos], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else if (ndefer < DEFER) {
+#pragma unroll
+          for (int d = 0; d < DEFER; ++d)
+            if (d == ndefer) deferred[d] = k + 1;
+          ndefer += 1;
+        } else {
+          set_error(w.status, ERR_RESIDUAL);
+        }
         // and this lane goes on with the left child
         ring = L;
         n = left;
@@ -418,7 +486,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
     int o = wave_claim32(w.status + ST_POLYS, finish);
     if (finish) {
       save_output(w, s, n, source, o);
-      atomicSub(w.control + C_PENDING, 1);
+      atomicSub(&sub_live, 1);
       task = -1;
     } else {
       // a child (or a task whose visit ran out of candidates) continues from
@@ -429,19 +497,21 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subdivide(Ws w) {
   }
   // Per-wave partial counters, summed by the next kernel: same-address
   // atomics from every wave at once would serialise at the queue's end.
-  long long part[SUB_COUNTERS] = {visits, tests, splits, f_straddle, b_straddle, f_front, b_front, 0};
-  for (int k = 0; k < SUB_COUNTERS - 1; ++k) part[k] = wave_sum(part[k]);
-  part[SUB_COUNTERS - 1] = wave_max(max_gen) + 1;
+  long long part[SUB_COUNTERS] = {visits, tests, splits, f_straddle, b_straddle, f_front, b_front, 0, 0};
+  for (int k = 0; k < SUB_SUMS; ++k) part[k] = wave_sum(part[k]);
+  part[SUB_SUMS] = wave_max(max_gen) + 1;
+  part[SUB_SUMS + 1] = wave_max(peak);
   const int wave_id = blockIdx.x * (SUB_THREADS / 64) + (threadIdx.x >> 6);
   if ((threadIdx.x & 63) == 0) {
     if (wave_id < SUB_PARTIAL_WAVES) {
       for (int k = 0; k < SUB_COUNTERS; ++k) w.sub_partials[(long)wave_id * SUB_COUNTERS + k] = part[k];
     } else {
-      const int slot[SUB_COUNTERS - 1] = {M_VISITS, M_CONTOUR_TESTS, M_SPLITS, M_FILTER_STRADDLE,
-                                          M_FALLBACK_STRADDLE, M_FILTER_FRONT, M_FALLBACK_FRONT};
-      for (int k = 0; k < SUB_COUNTERS - 1; ++k)
+      const int slot[SUB_SUMS] = {M_VISITS, M_CONTOUR_TESTS, M_SPLITS, M_FILTER_STRADDLE,
+                                  M_FALLBACK_STRADDLE, M_FILTER_FRONT, M_FALLBACK_FRONT};
+      for (int k = 0; k < SUB_SUMS; ++k)
         if (part[k]) atomicAdd((unsigned long long *)&w.metrics[slot[k]], (unsigned long long)part[k]);
-      atomicMax((unsigned long long *)&w.metrics[M_ROUNDS], (unsigned long long)part[SUB_COUNTERS - 1]);
+      atomicMax((unsigned long long *)&w.metrics[M_ROUNDS], (unsigned long long)part[SUB_SUMS]);
+      atomicMax(w.status + ST_PEAK, (int)part[SUB_SUMS + 1]);
     }
   }
 }

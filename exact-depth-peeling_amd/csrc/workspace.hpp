@@ -88,7 +88,6 @@ struct Ws {
   // frontier pool [MAXV][frontier]
   double2 *fr_xy;
   int *fr_n, *fr_src, *fr_node, *fr_gen;
-  int *fr_ready;  // queue entry q: 0 until published, then 1 + the task's frontier slot
   long long *sub_partials;  // [SUB_PARTIAL_WAVES][SUB_COUNTERS]
 
   // output polygons [MAXV][regions]
