@@ -144,6 +144,7 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
     c.take(w.tri_layer, T);
   }
   c.take(w.control, C_COUNT);
+  c.take(w.metric_rows, (long)METRIC_ROWS * METRIC_STRIDE);
   c.take(w.inverse, 16);
   c.take(w.canonical, nv1);
   c.take(w.vhash, w.vhash_size);

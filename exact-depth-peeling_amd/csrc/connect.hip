@@ -140,8 +140,8 @@ __device__ void match_one(const Ws &w, int i, int n) {
       }
     }
   }
-  wave_add_metric(&w.metrics[M_MATCHED], matched);
-  wave_add_metric(&w.metrics[M_HASH_MISSES], missed);
+  metric_add(w, M_MATCHED, matched);
+  metric_add(w, M_HASH_MISSES, missed);
 }
 
 __device__ int find_root(int *parents, int i) {
@@ -223,8 +223,8 @@ __device__ void connect_one(const Ws &w, int i, int n) {
       }
     }
   }
-  wave_add_metric(&w.metrics[M_CONNECTION_TESTS], visits);
-  wave_add_metric(&w.metrics[M_CONNECTIONS], linked);
+  metric_add(w, M_CONNECTION_TESTS, visits);
+  metric_add(w, M_CONNECTIONS, linked);
 }
 
 // roots (components.py:246-258)
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void k_roots(Ws w) {
     w.reps[i] = r;
     roots += r == i;
   }
-  wave_add_metric(&w.metrics[M_COMPONENTS], roots);
+  metric_add(w, M_COMPONENTS, roots);
 }
 
 static inline dim3 grid_for(long n, int block = 256) {
@@ -321,7 +321,7 @@ __global__ void k_merge(Ws w) {
       merged += 1;
     }
   }
-  wave_add_metric(&w.metrics[M_MERGED], merged);
+  metric_add(w, M_MERGED, merged);
 }
 
 __global__ void k_live_counts(Ws w) {

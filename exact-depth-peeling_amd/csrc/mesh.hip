@@ -171,7 +171,7 @@ __global__ void k_cell_insert(Ws w) {
       }
     }
   }
-  wave_add_metric(&w.metrics[M_UNIQUE_BVERTS], unique);
+  metric_add(w, M_UNIQUE_BVERTS, unique);
 }
 
 __device__ void cell_insert_one(const Ws &w, int i, int G) {
@@ -381,8 +381,7 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
     set_error(w.status, ERR_EDGE);
     valid = false;
   }
-  if ((threadIdx.x & 63) == 0 && top > 0)
-    atomicMax((unsigned long long *)&w.metrics[M_EDGE_MAX], (unsigned long long)top);
+  metric_max(w, M_EDGE_MAX, top);
   auto reg = [&](int k) {
     double v = p[0];
 #pragma unroll
@@ -461,7 +460,7 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
     });
     int total;
     int offset = wave_prefix(count, total);
-    if (lane == 0) atomicMax((unsigned long long *)&w.metrics[M_EDGE_MAX], (unsigned long long)total);
+    metric_max(w, M_EDGE_MAX, total);
     if (total > w.cap.edge_vertices) {
       if (lane == 0) set_error(w.status, ERR_EDGE);
       continue;
@@ -564,6 +563,21 @@ static inline dim3 grid_for(long n, int block = 256) {
   return dim3((unsigned)(g < 1 ? 1 : g));
 }
 
+// The metric rows folded into `metrics` (sums, and the maximum for the
+// largest edge candidate count), after every kernel that adds to them.
+__global__ void k_metrics_fold(Ws w) {
+  const int m = threadIdx.x;
+  if (m >= M_COUNT) return;
+  const bool maximum = m == M_EDGE_MAX;
+  long long acc = 0;
+#pragma unroll 8
+  for (int r = 0; r < METRIC_ROWS; ++r) {
+    long long x = w.metric_rows[(long)r * METRIC_STRIDE + m];
+    acc = maximum ? max(acc, x) : acc + x;
+  }
+  w.metrics[m] = maximum ? max(w.metrics[m], acc) : w.metrics[m] + acc;
+}
+
 void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t) {
   // ring offsets: an exclusive scan of the polygons' vertex counts
   scan_exclusive(s, w.out_n, w.voffsets, w.status + ST_POLYS, 0, w.cap.regions, w.control + C_NBV, w.scan);
@@ -576,6 +590,7 @@ void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t) {
   mark(t, s, "k_mesh_edges");
   hipLaunchKernelGGL(k_mesh_crowded, dim3(1024), dim3(256), 0, s, w);
   mark(t, s, "k_mesh_crowded");
+  hipLaunchKernelGGL(k_metrics_fold, dim3(1), dim3(64), 0, s, w);
 }
 
 }  // namespace peel

@@ -118,6 +118,10 @@ enum Control : int {
 // kernel: sums (visits, contour tests, splits, the four binary32 filter
 // counts), then maxima (rounds, the peak of a block's live tasks).
 constexpr int SUB_SUMS = 7;
+// rows of metric partials (a power of two) and their stride in words (two
+// 128-byte lines, at least M_COUNT)
+constexpr int METRIC_ROWS = 64;
+constexpr int METRIC_STRIDE = 32;
 constexpr int SUB_COUNTERS = SUB_SUMS + 2;
 constexpr int SUB_PARTIAL_WAVES = 8192;
 

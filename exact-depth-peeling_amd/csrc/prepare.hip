@@ -21,6 +21,7 @@ __global__ void k_reset(Ws w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < 16) w.status[i] = 0;
   if (i < M_COUNT) w.metrics[i] = 0;
+  for (int j = i; j < METRIC_ROWS * METRIC_STRIDE; j += gridDim.x * blockDim.x) w.metric_rows[j] = 0;
   if (i < C_COUNT) w.control[i] = 0;
   for (int j = i; j < w.vhash_size; j += gridDim.x * blockDim.x) w.vhash[j] = -1;
   for (int j = i; j < w.ehash_size; j += gridDim.x * blockDim.x) w.etable[j] = EdgeSlot{~0ull, 0ull};
@@ -625,7 +626,7 @@ __global__ void k_segments_reduce(Ws w) {
     if (__hip_atomic_load(&w.seg_near[w.nseg + rep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > kb)
       atomicMin(&w.seg_near[w.nseg + rep], kb);
   }
-  if (valid && rep == i) atomicAdd((unsigned long long *)&w.metrics[M_UNIQUE], 1ull);
+  metric_add(w, M_UNIQUE, valid && rep == i ? 1 : 0);
 }
 
 __global__ void k_segment_flags(Ws w) {

@@ -303,9 +303,9 @@ __global__ __launch_bounds__(256) void k_certify(Ws w) {
     while (node < tree.leaves) node *= 2;
   int slot = wave_claim32(w.control + C_BEAM_JOBS, handoff);
   if (handoff) w.beam_jobs[slot] = make_int2(c, node);
-  wave_add_metric(&w.metrics[M_BEAM_ATTEMPTS], ok);
-  wave_add_metric(&w.metrics[M_BEAM_TESTS], visits);
-  wave_add_metric(&w.metrics[M_HEAVY_BEAMS], handoff);
+  metric_add(w, M_BEAM_ATTEMPTS, ok);
+  metric_add(w, M_BEAM_TESTS, visits);
+  metric_add(w, M_HEAVY_BEAMS, handoff);
 }
 
 // certify_heavy (beams.py:150-189): the unvisited suffix of each handed-off
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_certify_heavy(Ws w, int workers) {
       }
     }
   }
-  wave_add_metric(&w.metrics[M_BEAM_TESTS], visits);
+  metric_add(w, M_BEAM_TESTS, visits);
 }
 
 // apply (beams.py:192-207): every patch of a certified cell joins its leader
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(256) void k_beam_apply(Ws w) {
       }
     }
   }
-  wave_add_metric(&w.metrics[M_BEAMS], certified);
+  metric_add(w, M_BEAMS, certified);
 }
 
 // Source rings in hierarchy leaf order: leaf k holds its ring as one 128-byte
@@ -689,12 +689,12 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
     }
   }
   flush_hits(w, stage);
-  wave_add_metric(&w.metrics[M_RAY_GROUPS], groups);
-  wave_add_metric(&w.metrics[M_RAY_TESTS], visits_all);
-  wave_add_metric(&w.metrics[M_HEAVY], heavy);
+  metric_add(w, M_RAY_GROUPS, groups);
+  metric_add(w, M_RAY_TESTS, visits_all);
+  metric_add(w, M_HEAVY, heavy);
   if (w.filter32) {
-    wave_add_metric(&w.metrics[M_FILTER_TESTS], filtered);
-    wave_add_metric(&w.metrics[M_FALLBACKS], fallbacks);
+    metric_add(w, M_FILTER_TESTS, filtered);
+    metric_add(w, M_FALLBACKS, fallbacks);
   }
 }
 
@@ -779,10 +779,10 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
     }
   }
   flush_hits(w, stage);
-  wave_add_metric(&w.metrics[M_RAY_TESTS], visits);
+  metric_add(w, M_RAY_TESTS, visits);
   if (w.filter32) {
-    wave_add_metric(&w.metrics[M_FILTER_TESTS], filtered);
-    wave_add_metric(&w.metrics[M_FALLBACKS], fallbacks);
+    metric_add(w, M_FILTER_TESTS, filtered);
+    metric_add(w, M_FALLBACKS, fallbacks);
   }
 }
 
@@ -1170,8 +1170,8 @@ __global__ __launch_bounds__(256) void k_assign(Ws w) {
   top = wave_max(top);
   if ((threadIdx.x & 63) == 0 && top > 0) atomicMax(w.status + ST_LAYERS, top);
   if (w.filter32) {
-    wave_add_metric(&w.metrics[M_FILTER_DEPTH], filtered);
-    wave_add_metric(&w.metrics[M_FALLBACK_DEPTH], fallbacks);
+    metric_add(w, M_FILTER_DEPTH, filtered);
+    metric_add(w, M_FALLBACK_DEPTH, fallbacks);
   }
 }
 

@@ -163,6 +163,10 @@ struct Ws {
   int *pool_n;              // [3][pieces + 3][workers]
   long long *stats;         // [cap.layers][3] targets, candidate tests, fully hidden
 
+  // metric partials: wave v adds to row v % METRIC_ROWS (rows on lines of
+  // their own), folded into `metrics` by the pipeline's last kernel
+  long long *metric_rows;   // [METRIC_ROWS][METRIC_STRIDE]
+
   SortScratch sort;
   ScanScratch scan;
   int persistent_blocks;    // grid of the persistent kernels
@@ -177,6 +181,24 @@ struct Ws {
   // launch with grid barriers (set for graphs whose members run concurrently)
   int split_huge;
 };
+
+// Wave-level metric updates. Device atomics on one address serialise (about
+// 12 ns each on MI355X, 6 ns on one line), so thousands of waves adding to
+// the same metric word would queue behind each other; spread over
+// METRIC_ROWS lines they cost next to nothing. Called by every lane of the
+// wave (the value is summed, or maximised, across the wave first).
+__device__ __forceinline__ long long *metric_row(const Ws &w) {
+  int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  return w.metric_rows + (long)(wave & (METRIC_ROWS - 1)) * METRIC_STRIDE;
+}
+__device__ __forceinline__ void metric_add(const Ws &w, int m, long long v) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long *)&metric_row(w)[m], (unsigned long long)v);
+}
+// `v` already uniform across the wave; lane 0 records it
+__device__ __forceinline__ void metric_max(const Ws &w, int m, long long v) {
+  if ((threadIdx.x & 63) == 0 && v > 0) atomicMax((unsigned long long *)&metric_row(w)[m], (unsigned long long)v);
+}
 
 // Optional host-side timeline: hipEvents recorded between stages when the
 // pipeline runs eagerly for a stage profile (never inside a capture).
