@@ -95,7 +95,7 @@ int peel::plan(Ws &w, const PeelDesc *d) {
   for (int s : {w.vhash_size, w.ehash_size, w.shash_size, w.qhash_size, w.ehash2_size, w.phash_size,
                 w.source_tree.size, w.contour_tree.size})
     if (s == 0) return fail(2, "mesh or capacities too large: a hash table or hierarchy would exceed 2^30 slots");
-  w.cell_capacity = 4 * BV;
+  w.cell_capacity = std::min<long>(4 * BV, 1L << 30);
   auto bits_for = [](long n) {
     int b = 1;
     while ((1L << b) < n) ++b;
@@ -106,7 +106,7 @@ int peel::plan(Ws &w, const PeelDesc *d) {
   long sort_cap = std::max<long>({(long)w.cap.hits, (long)w.source_tree.size, (long)w.contour_tree.size,
                                   (long)w.cap.regions, layered ? (long)w.cap.frontier : 1L});
   w.sort.max_blocks = (int)((sort_cap + RS_TILE - 1) / RS_TILE);
-  long scan_cap = std::max<long>(std::max<long>(w.cap.regions, BV), w.nseg);
+  long scan_cap = std::max<long>({(long)w.cap.regions, BV, (long)w.nseg, w.cell_capacity});
   w.scan.max_blocks = (int)((scan_cap + 4095) / 4096) + 1;
   w.ray_lanes_target = 524288;
   w.heavy_lanes = 65536;
@@ -249,8 +249,9 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
   c.take(w.tasks, BV);
   c.take(w.crowded, BV);
   c.take(w.phash, w.phash_size);
-  c.take(w.cell_heads, w.cell_capacity);
-  c.take(w.cell_next, BV);
+  c.take(w.cell_start, w.cell_capacity);
+  c.take(w.cell_of, BV);
+  c.take(w.cell_xy, BV);
   c.take(w.sort.counts, RS_MAX_SWEEPS * 256);
   c.take(w.sort.tickets, RS_MAX_SWEEPS);
   c.take(w.sort.tiles, (long)RS_MAX_SWEEPS * w.sort.max_blocks * 256);

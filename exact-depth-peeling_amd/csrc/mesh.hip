@@ -79,13 +79,16 @@ __device__ int grid_side(const Ws &w);
 __device__ __forceinline__ long cells_live(const Ws &w, int G);
 
 // Deduplicate the boundary vertices per layer; beside it, plan the per-layer
-// cell grid from the vertex total and clear its live cells for k_cell_insert.
+// cell grid from the vertex total and clear its live cells' counts.
 __global__ void k_point_dedupe(Ws w) {
   int G = grid_side(w);
-  if (blockIdx.x == 0 && threadIdx.x == 0) w.control[C_GRID] = G;
   long cells = cells_live(w, G);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    w.control[C_GRID] = G;
+    w.control[C_CELLS] = (int)cells;
+  }
   for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < cells; j += (long)gridDim.x * blockDim.x)
-    w.cell_heads[j] = -1;
+    w.cell_start[j] = 0;
   if (w.status[ST_ERR] != 0) return;
   ITEM_LOOP(i, load_i(w.status + ST_BVERTS)) dedupe_one(w, i);
 }
@@ -157,27 +160,35 @@ __device__ __forceinline__ long cells_live(const Ws &w, int G) {
   return (long)max(1, w.status[ST_LAYERS]) * G * G;
 }
 
-__device__ void cell_insert_one(const Ws &w, int i, int G);
-
-// Link every first copy of a (layer, point) into its cell's list.
-__global__ void k_cell_insert(Ws w) {
+// Count every first copy of a (layer, point) in its cell (and remember the
+// cell); an exclusive scan of the counts gives each cell's first row, and
+// k_cell_fill stores the points cell by cell.
+__global__ void k_cell_count(Ws w) {
   long long unique = 0;
   if (w.status[ST_ERR] == 0) {
     int G = load_i(w.control + C_GRID);
     ITEM_LOOP(i, load_i(w.status + ST_BVERTS)) {
+      int slot = -1;
       if (first_copy(w, i)) {
-        cell_insert_one(w, i, G);
+        double2 x = w.bverts[i];
+        slot = (int)cell_slot(w.blayers[i], cell_index(x.x, G), cell_index(x.y, G), G);
+        atomicAdd(w.cell_start + slot, 1);
         unique += 1;
       }
+      w.cell_of[i] = slot;
     }
   }
   metric_add(w, M_UNIQUE_BVERTS, unique);
 }
 
-__device__ void cell_insert_one(const Ws &w, int i, int G) {
-  double2 x = w.bverts[i];
-  long slot = cell_slot(w.blayers[i], cell_index(x.x, G), cell_index(x.y, G), G);
-  w.cell_next[i] = atomicExch(w.cell_heads + slot, i);
+// Each cell's first row advances past its points as they are stored, so a
+// filled cell c ends at cell_start[c] and starts where cell c - 1 ends.
+__global__ void k_cell_fill(Ws w) {
+  if (w.status[ST_ERR] != 0) return;
+  ITEM_LOOP(i, load_i(w.status + ST_BVERTS)) {
+    int slot = w.cell_of[i];
+    if (slot >= 0) w.cell_xy[atomicAdd(w.cell_start + slot, 1)] = w.bverts[i];
+  }
 }
 
 // Visit every distinct vertex of `layer` in a cell that can hold a point within
@@ -202,9 +213,12 @@ __device__ __forceinline__ void walk_segment(const Ws &w, double2 p0, double2 p1
       yhi = fmin(yb, fmax(y0, y1));
     }
     int cy0 = cell_index(ylo - NODE_PAD, G), cy1 = cell_index(yhi + NODE_PAD, G);
-    for (int cy = cy0; cy <= cy1; ++cy)
-      for (int k = w.cell_heads[cell_slot(layer, cx, cy, G)]; k >= 0; k = w.cell_next[k])
-        if (!visit(k)) return;
+    for (int cy = cy0; cy <= cy1; ++cy) {
+      long slot = cell_slot(layer, cx, cy, G);
+      int b = slot > 0 ? w.cell_start[slot - 1] : 0, e = w.cell_start[slot];
+      for (int k = b; k < e; ++k)
+        if (!visit(w.cell_xy[k])) return;
+    }
   }
 }
 
@@ -355,9 +369,9 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
     if (box_cells(q.a, q.b, G) > w.long_edge_cells) {
       crowded = true;  // long edges go straight to the wave-parallel path
     } else {
-      walk_segment(w, q.a, q.b, layer, G, [&](int k) {
+      walk_segment(w, q.a, q.b, layer, G, [&](double2 x) {
         double t;
-        if (q.accept(w.bverts[k], t)) {
+        if (q.accept(x, t)) {
 #pragma unroll
           for (int r = 0; r < REG_PARAMS; ++r) p[r] = (r == count) ? t : p[r];
           count += 1;
@@ -439,9 +453,9 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
     const int G = load_i(w.control + C_GRID);
     double mine[LANE_PARAMS];
     int count = 0;
-    if (q.length2 > 0.0) walk_segment(w, p0, p1, layer, G, [&](int k) {
+    if (q.length2 > 0.0) walk_segment(w, p0, p1, layer, G, [&](double2 x) {
       double tt;
-      if (q.accept(w.bverts[k], tt) && tt >= t0 && (tt < t1 || lane == 63)) {
+      if (q.accept(x, tt) && tt >= t0 && (tt < t1 || lane == 63)) {
         // insertion into the lane's sorted run (static indices only)
         if (count < LANE_PARAMS) {
 #pragma unroll
@@ -472,9 +486,9 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
       // a lane with a dense sub-range gathers again straight into LDS and
       // insertion-sorts its run there
       int m = 0;
-      walk_segment(w, p0, p1, layer, G, [&](int k) {
+      walk_segment(w, p0, p1, layer, G, [&](double2 x) {
         double tt;
-        if (q.accept(w.bverts[k], tt) && tt >= t0 && (tt < t1 || lane == 63)) list[offset + m++] = tt;
+        if (q.accept(x, tt) && tt >= t0 && (tt < t1 || lane == 63)) list[offset + m++] = tt;
         return true;
       });
       for (int a = 1; a < m; ++a) {
@@ -584,7 +598,9 @@ void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t) {
   hipLaunchKernelGGL(k_scatter, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   hipLaunchKernelGGL(k_point_dedupe, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "scatter_vertices");
-  hipLaunchKernelGGL(k_cell_insert, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
+  hipLaunchKernelGGL(k_cell_count, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
+  scan_exclusive(s, w.cell_start, w.cell_start, w.control + C_CELLS, 0, (int)w.cell_capacity, nullptr, w.scan);
+  hipLaunchKernelGGL(k_cell_fill, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "vertex_grid");
   hipLaunchKernelGGL(k_mesh_edges, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "k_mesh_edges");

@@ -106,6 +106,7 @@ enum Control : int {
   C_LOOP = 140,      // layered route: another peeling pass is due
   C_LEADERS = 141,   // ray-group leaders (they sort first in the ray order)
   C_GRID = 142,      // cells per axis of the boundary-vertex grid
+  C_CELLS = 146,     // live cells of that grid (layers * G^2)
   C_LIVE = 143,      // patches left after convex coalescing
   C_SORTN = 144,     // layered route: residual pieces to order (0 once the loop ends)
   C_RAYSORT = 145,   // leaders to put in Morton order (0 when a wave walks each ray)

@@ -135,9 +135,12 @@ struct Ws {
   int2 *tasks;              // [vertices]
   int *crowded;             // [vertices] edges for the per-lane-array noding path
   int *phash;               // [phash_size] boundary vertex dedupe (layer, x, y)
-  int *cell_heads;          // [cell_capacity] first vertex of each (layer, cell) list
-  long cell_capacity;       // 4 * vertices: layers * G^2 never exceeds it
-  int *cell_next;           // [vertices] next vertex in the same cell
+  // per-layer cell grid of the distinct boundary vertices, compressed rows:
+  // cell c holds cell_xy[cell_start[c - 1], cell_start[c]) once it is filled
+  int *cell_start;          // [cell_capacity] counts, then (scanned and filled) row ends
+  long cell_capacity;       // 4 * vertices (at most 2^30): layers * G^2 never exceeds it
+  int *cell_of;             // [vertices] cell of a first copy, -1 for a repeat
+  double2 *cell_xy;         // [vertices] the distinct vertices, cell by cell
   int phash_size;
   double *tri_uv;           // [triangles][3][2]
   double *tri_world;        // [triangles][3][3]
