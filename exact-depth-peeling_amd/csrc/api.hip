@@ -122,7 +122,6 @@ static void carve_hierarchy(Carver &c, Hierarchy &h) {
   c.take(h.alt_ids, h.size);
   c.take(h.nodes, 2L * h.size + 4);  // four boxes of slack for group reads
   c.take(h.leaves, 1);
-  c.take(h.level_counters, HIER_COUNTERS);
 }
 
 // Every pool of a planned workspace. `outputs` false leaves the caller-owned

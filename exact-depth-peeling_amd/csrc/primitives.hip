@@ -442,53 +442,35 @@ __device__ __forceinline__ Box unite(const Box &a, const Box &b) {
   return r;
 }
 
-// Every internal level in one launch. Each block reduces a group of up to 512
-// nodes of the bottom level in LDS, nine levels at a time; the block that
-// completes a group of 512 such results (an arrival counter, release and
-// acquire fences around it) goes on to reduce that group, and so on to the
-// root. Each stage's counters are cleared by the block that goes on.
-__global__ __launch_bounds__(256) void h_levels(Box *nodes, const int *leaves, int *counters) {
+// The internal levels, nine per launch: each block reduces a group of up to
+// 512 nodes of the stage's bottom level in LDS, nine levels at a time, and the
+// next stage starts from the nodes this one wrote. Stages are separate
+// launches (in-kernel hand-offs between blocks would need agent-scope fences,
+// which slow every wave's loads); a stage above the root returns at once.
+__global__ __launch_bounds__(256) void h_levels(Box *nodes, const int *leaves, int stage) {
   __shared__ Box lds[HIER_GROUP];
-  __shared__ int proceed;
-  int bottom = *leaves;
-  int group = blockIdx.x;
-  int width = bottom < HIER_GROUP ? bottom : HIER_GROUP;
+  const int bottom = *leaves >> (HIER_STAGE_LEVELS * stage);
+  const int group = blockIdx.x;
+  const int width = bottom < HIER_GROUP ? bottom : HIER_GROUP;
   if (bottom <= 1 || group * width >= bottom) return;
-  for (int stage = 0;; ++stage) {
-    int levels = 0;
-    while ((1 << (levels + 1)) <= width) ++levels;
-    int first = bottom + group * width;
-    for (int i = threadIdx.x; i < width; i += 256) lds[i] = nodes[first + i];
+  int levels = 0;
+  while ((1 << (levels + 1)) <= width) ++levels;
+  const int first = bottom + group * width;
+  for (int i = threadIdx.x; i < width; i += 256) lds[i] = nodes[first + i];
+  __syncthreads();
+  int count = width, start = first;
+  for (int l = 0; l < levels; ++l) {
+    int half = count >> 1;
+    Box parent;
+    if ((int)threadIdx.x < half) parent = unite(lds[2 * threadIdx.x], lds[2 * threadIdx.x + 1]);
     __syncthreads();
-    int count = width, start = first;
-    for (int l = 0; l < levels; ++l) {
-      int half = count >> 1;
-      Box parent;
-      if ((int)threadIdx.x < half) parent = unite(lds[2 * threadIdx.x], lds[2 * threadIdx.x + 1]);
-      __syncthreads();
-      start >>= 1;
-      if ((int)threadIdx.x < half) {
-        lds[threadIdx.x] = parent;
-        nodes[start + threadIdx.x] = parent;
-      }
-      __syncthreads();
-      count = half;
+    start >>= 1;
+    if ((int)threadIdx.x < half) {
+      lds[threadIdx.x] = parent;
+      nodes[start + threadIdx.x] = parent;
     }
-    int top = bottom / width;  // nodes of the level this stage ended on
-    if (top <= 1) return;
-    int next_width = top < HIER_GROUP ? top : HIER_GROUP;
-    int next = group / next_width;
-    int *counter = counters + stage * HIER_GROUP + next;
-    release_fence();
     __syncthreads();
-    if (threadIdx.x == 0) proceed = atomicAdd(counter, 1) == next_width - 1;
-    __syncthreads();
-    if (!proceed) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (threadIdx.x == 0) *counter = 0;
-    bottom = top;
-    group = next;
-    width = next_width;
+    count = half;
   }
 }
 
@@ -503,9 +485,11 @@ void build_hierarchy(hipStream_t s, const Box *boxes, const int *count, int coun
              scratch);
   hipLaunchKernelGGL(h_leaves, dim3((h.size + 255) / 256), dim3(256), 0, s, boxes, count,
                      count_static, cap, h.ids, h.leaves, h.nodes, gather);
-  long blocks = h.size / HIER_GROUP;
-  hipLaunchKernelGGL(h_levels, dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(256), 0, s, h.nodes, h.leaves,
-                     h.level_counters);
+  for (int stage = 0; (1L << (HIER_STAGE_LEVELS * stage)) < h.size; ++stage) {
+    long blocks = (h.size >> (HIER_STAGE_LEVELS * stage)) / HIER_GROUP;
+    hipLaunchKernelGGL(h_levels, dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(256), 0, s, h.nodes, h.leaves,
+                       stage);
+  }
 }
 
 }  // namespace peel

@@ -47,14 +47,13 @@ void scan_exclusive(hipStream_t s, const int *in, int *out, const int *count, in
 // captured graph builds and traverses only the live part of a pool whose
 // capacity is `size`.
 constexpr int HIER_GROUP = 512;                   // nodes reduced by one block per stage
-constexpr int HIER_COUNTERS = 4 * HIER_GROUP;     // arrival counters of up to four stages above the first
+constexpr int HIER_STAGE_LEVELS = 9;              // levels one stage reduces (log2 HIER_GROUP)
 
 struct Hierarchy {
   uint64_t *keys = nullptr, *alt_keys = nullptr;
   int *ids = nullptr, *alt_ids = nullptr;  // [size]
   Box *nodes = nullptr;                     // [2 * size]
   int *leaves = nullptr;                    // device: current leaf count
-  int *level_counters = nullptr;            // [HIER_COUNTERS] zero between builds
   int size = 1;                             // capacity (power of two)
   __device__ Tree tree() const { return Tree{nodes, ids, __hip_atomic_load(leaves, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)}; }
 };

@@ -602,12 +602,13 @@ __global__ __launch_bounds__(256) void k_ray_keys(Ws w) {
     }
     __syncthreads();
   }
-  release_fence();
+  // Each block's leader count is an atomic that has returned before its
+  // arrival is counted, so the last block reads the total from the counter
+  // itself: no fence is needed (the keys reach the sort at the kernel's end).
   __syncthreads();
   if (threadIdx.x == 0) last = atomicAdd(w.control + C_RAYKEY_DONE, 1) == (int)gridDim.x - 1;
   __syncthreads();
   if (last && threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     int leaders = load_i(w.control + C_LEADERS);
     w.control[C_RAYSORT] = ray_lanes(w.source_tree.tree(), leaders, w.ray_lanes_target) == 1 ? leaders : 0;
     w.control[C_RAYKEY_DONE] = 0;
@@ -1168,7 +1169,8 @@ __global__ __launch_bounds__(256) void k_assign(Ws w) {
     ITEM_LOOP(i, npolys(w)) top = max(top, assign_one(w, i, filtered, fallbacks));
   }
   top = wave_max(top);
-  if ((threadIdx.x & 63) == 0 && top > 0) atomicMax(w.status + ST_LAYERS, top);
+  // most waves see the layer count already at least their top: no atomic
+  if ((threadIdx.x & 63) == 0 && top > load_i(w.status + ST_LAYERS)) atomicMax(w.status + ST_LAYERS, top);
   if (w.filter32) {
     metric_add(w, M_FILTER_DEPTH, filtered);
     metric_add(w, M_FALLBACK_DEPTH, fallbacks);
