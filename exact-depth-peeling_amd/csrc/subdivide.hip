@@ -57,14 +57,15 @@ __device__ __forceinline__ double2 vtx(const Slot &s, int j) { return s.xy[(long
 // copied to its slot). The first SUB_LDS vertices of a ring live in LDS,
 // vertex-major so the 64 lanes of a wave read vertex j of their rings from
 // consecutive banks; the rare longer ring keeps the rest in its slot (slots
-// are never reused within a run). 576 B of LDS per lane: one block per CU.
+// are never reused within a run). 288 B of LDS per lane (six vertices a
+// ring: measured faster than twelve on both the torus knot and the shells).
 constexpr int SUB_THREADS = 256;
-constexpr int SUB_LDS = 12;
+constexpr int SUB_LDS = 6;
 constexpr int SUB_RING = SUB_LDS * SUB_THREADS;  // LDS words of one ring for the block
 __shared__ double2 sub_ring_lds[3 * SUB_RING];
 // The block's queue of right children (frontier slots; -1: empty), a ring of
 // BQ entries with monotonic head and tail, and the block's live tasks.
-constexpr int BQ = 2048;
+constexpr int BQ = 1024;
 constexpr int BQ_MARGIN = SUB_THREADS;  // room for every lane of the block to push at once
 __shared__ int sub_queue[BQ];
 __shared__ int sub_head, sub_tail, sub_live;
