@@ -128,14 +128,15 @@ __device__ __forceinline__ bool first_copy(const Ws &w, int i) {
 
 // ---------------------------------------------------------------------------
 // Cell grid of the distinct boundary vertices, per layer: G x G cells over the
-// unit window (G a power of two chosen on the device for about one distinct
-// vertex per cell), a dense array of list heads indexed by (layer, cell), and
-// one link per vertex. Built in two kernels without sorting; an edge query
-// walks only the cells its segment passes through (instead of a layered
-// hierarchy over vertex boxes, which a long diagonal edge meets over its whole
-// bounding box). layers * G^2 stays within the allocated heads.
+// unit window (G a power of two chosen on the device), cells numbered column
+// by column, and the vertices stored cell by cell in that order (compressed
+// rows: a count per cell, an exclusive scan, a fill). The cells of one column
+// between two rows therefore hold one contiguous run of vertices, so an edge
+// query reads one run per column its segment passes through (instead of a
+// layered hierarchy over vertex boxes, which a long diagonal edge meets over
+// its whole bounding box). layers * G^2 stays within the allocated cells.
 __device__ __forceinline__ long cell_slot(int layer, int cx, int cy, int G) {
-  return ((long)layer * G + cy) * G + cx;
+  return ((long)layer * G + cx) * G + cy;
 }
 
 __device__ __forceinline__ int cell_index(double v, int G) {
@@ -196,15 +197,42 @@ __global__ void k_cell_fill(Ws w) {
 // each row that the segment's y-range over the padded column (itself padded)
 // touches. A vertex accepted by the edge test lies within EPS of the segment,
 // far inside PAD, so it is visited; each cell, and so each vertex, once.
+// A column's rows are one run of vertices, read SCAN_BATCH at a time with the
+// loads issued together (one memory latency per batch, not per vertex).
 constexpr double NODE_PAD = 2.0e-7;
+constexpr int SCAN_BATCH = 4;
 
 template <class Visit>
-__device__ __forceinline__ void walk_segment(const Ws &w, double2 p0, double2 p1, int layer, int G, Visit visit) {
-  double xa = fmin(p0.x, p1.x), xb = fmax(p0.x, p1.x);
-  double ya = fmin(p0.y, p1.y), yb = fmax(p0.y, p1.y);
-  double dx = p1.x - p0.x, dy = p1.y - p0.y;
-  int cx0 = cell_index(xa - NODE_PAD, G), cx1 = cell_index(xb + NODE_PAD, G);
-  for (int cx = cx0; cx <= cx1; ++cx) {
+__device__ __forceinline__ bool scan_run(const Ws &w, int b, int e, Visit &visit) {
+  for (int k = b; k < e; k += SCAN_BATCH) {
+    double2 x[SCAN_BATCH];
+    // past the run's end the batch repeats its last vertex, which is skipped
+#pragma unroll
+    for (int r = 0; r < SCAN_BATCH; ++r) x[r] = w.cell_xy[min(k + r, e - 1)];
+#pragma unroll
+    for (int r = 0; r < SCAN_BATCH; ++r)
+      if (k + r < e && !visit(x[r])) return false;
+  }
+  return true;
+}
+
+// The columns a walk over segment p0-p1 passes through, and in column cx the
+// run of cell_xy rows [b, e) of the rows it touches there.
+struct SegmentCells {
+  double2 p0;
+  double xa, xb, ya, yb, dx, dy;
+  int layer, G, cx0, cx1;
+  __device__ SegmentCells(double2 a, double2 b, int layer_, int G_) : p0(a), layer(layer_), G(G_) {
+    xa = fmin(a.x, b.x);
+    xb = fmax(a.x, b.x);
+    ya = fmin(a.y, b.y);
+    yb = fmax(a.y, b.y);
+    dx = b.x - a.x;
+    dy = b.y - a.y;
+    cx0 = cell_index(xa - NODE_PAD, G);
+    cx1 = cell_index(xb + NODE_PAD, G);
+  }
+  __device__ int2 run(const Ws &w, int cx) const {
     double xl = fmax(xa, (double)cx / G - NODE_PAD), xr = fmin(xb, (double)(cx + 1) / G + NODE_PAD);
     double ylo = ya, yhi = yb;
     if (dx != 0.0 && xl <= xr) {
@@ -213,12 +241,17 @@ __device__ __forceinline__ void walk_segment(const Ws &w, double2 p0, double2 p1
       yhi = fmin(yb, fmax(y0, y1));
     }
     int cy0 = cell_index(ylo - NODE_PAD, G), cy1 = cell_index(yhi + NODE_PAD, G);
-    for (int cy = cy0; cy <= cy1; ++cy) {
-      long slot = cell_slot(layer, cx, cy, G);
-      int b = slot > 0 ? w.cell_start[slot - 1] : 0, e = w.cell_start[slot];
-      for (int k = b; k < e; ++k)
-        if (!visit(w.cell_xy[k])) return;
-    }
+    long first = cell_slot(layer, cx, cy0, G), last = cell_slot(layer, cx, cy1, G);
+    return make_int2(first > 0 ? w.cell_start[first - 1] : 0, w.cell_start[last]);
+  }
+};
+
+template <class Visit>
+__device__ __forceinline__ void walk_segment(const Ws &w, double2 p0, double2 p1, int layer, int G, Visit visit) {
+  const SegmentCells cells(p0, p1, layer, G);
+  for (int cx = cells.cx0; cx <= cells.cx1; ++cx) {
+    int2 r = cells.run(w, cx);
+    if (!scan_run(w, r.x, r.y, visit)) return;
   }
 }
 
@@ -247,16 +280,20 @@ constexpr int EDGE_MAX = 128;
 
 // Parameters of the same-layer vertices strictly inside edge (a, b):
 // t in (EPS, 1 - EPS) and distance <= EPS * |b - a| (kernels.py:1040-1064).
+// The distance test, a few products, goes first: almost every vertex a walk
+// reads is off the line, and t (a division) is formed only for the others.
 struct EdgeQuery {
   double2 a, b, d;
   double length2, len;
   __device__ bool accept(double2 x, double &t) const {
+    if (!(fabs(cross3(a, b, x)) <= EPS * len)) return false;
     t = ((x.x - a.x) * d.x + (x.y - a.y) * d.y) / length2;
-    return t > EPS && t < 1.0 - EPS && fabs(cross3(a, b, x)) <= EPS * len;
+    return t > EPS && t < 1.0 - EPS;
   }
 };
 
 constexpr int REG_PARAMS = 8;
+constexpr int DENSE_ROWS = 128;
 
 // Sorting network on 8 registers (Batcher odd-even merge), static indices only.
 __device__ __forceinline__ void cswap(double &x, double &y) {
@@ -366,22 +403,34 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
   if (valid) valid = edge_setup(w, j, q, center, i, layer);
   if (valid && q.length2 > 0.0) {
     const int G = load_i(w.control + C_GRID);
-    if (box_cells(q.a, q.b, G) > w.long_edge_cells) {
-      crowded = true;  // long edges go straight to the wave-parallel path
-    } else {
-      walk_segment(w, q.a, q.b, layer, G, [&](double2 x) {
-        double t;
-        if (q.accept(x, t)) {
+    const SegmentCells cells(q.a, q.b, layer, G);
+    // long edges, and edges through more than DENSE_ROWS vertices, go
+    // straight to the wave-parallel path
+    crowded = box_cells(q.a, q.b, G) > w.long_edge_cells;
+    if (!crowded) {
+      int rows = 0;
+      for (int cx = cells.cx0; cx <= cells.cx1; ++cx) {
+        int2 r = cells.run(w, cx);
+        rows += max(r.y - r.x, 0);
+      }
+      crowded = rows > DENSE_ROWS;
+    }
+    auto visit = [&](double2 x) {
+      double t;
+      if (q.accept(x, t)) {
 #pragma unroll
-          for (int r = 0; r < REG_PARAMS; ++r) p[r] = (r == count) ? t : p[r];
-          count += 1;
-          if (count > REG_PARAMS) {
-            crowded = true;
-            return false;
-          }
+        for (int r = 0; r < REG_PARAMS; ++r) p[r] = (r == count) ? t : p[r];
+        count += 1;
+        if (count > REG_PARAMS) {
+          crowded = true;
+          return false;
         }
-        return true;
-      });
+      }
+      return true;
+    };
+    for (int cx = cells.cx0; !crowded && cx <= cells.cx1; ++cx) {
+      int2 r = cells.run(w, cx);
+      if (!scan_run(w, r.x, r.y, visit)) break;
     }
     sort8(p);
   }
@@ -425,19 +474,21 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
   fan(w, q, center, count, reg, true, base + offset, source, layer, sc);
 }
 
-// Crowded edges, one wave each. Lane l owns the parameter range
-// [l/64, (l+1)/64) of the edge and queries only the box of that sub-segment
-// (padded like the whole-edge box, so every vertex the whole-edge query would
-// accept is found by exactly one lane). The lanes' sorted runs concatenate into
-// the sorted parameter list in LDS; lane 0 then walks the fan. Up to
-// edge_vertices parameters; more sets error bit 128 (kernels.py:1051-1057).
-constexpr int LANE_PARAMS = 16;
-
+// Crowded edges, one wave each, the whole edge read by all its lanes: the
+// walk's column runs (one per column, up to 64 columns at a time, a column per
+// lane) are laid end to end, and the lanes take the rows of that sequence 64 at
+// a time, so a dense cell or a long edge is spread over the wave. Accepted
+// parameters are appended to a list in LDS and ranked into sorted order there;
+// the fan is then built from the sorted list. Up to edge_vertices parameters;
+// more sets error bit 128 (kernels.py:1051-1057).
 __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
-  __shared__ double lds[4][EDGE_MAX];
+  __shared__ double found_lds[4][EDGE_MAX];  // accepted parameters, in the order found
+  __shared__ double lds[4][EDGE_MAX];        // the same, sorted
   __shared__ int kept_lds[4][EDGE_MAX + 2];  // fan steps the skip rule keeps, then their count
+  __shared__ int run_row[4][64], run_at[4][64];  // first row of each lane's run, its place in the sequence
   int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double *list = lds[wave];
+  double *list = lds[wave], *found = found_lds[wave];
+  int *row0 = run_row[wave], *at = run_at[wave];
   int n = load_i(w.control + C_CROWDED);
   if (n > w.cap.vertices) n = w.cap.vertices;
   int waves = gridDim.x * 4;
@@ -447,59 +498,61 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
     double2 center;
     int i, layer;
     if (!edge_setup(w, j, q, center, i, layer)) continue;  // wave-uniform
-    double t0 = lane / 64.0, t1 = (lane + 1) / 64.0;
-    double2 p0 = make_double2(q.a.x + t0 * q.d.x, q.a.y + t0 * q.d.y);
-    double2 p1 = make_double2(q.a.x + t1 * q.d.x, q.a.y + t1 * q.d.y);
     const int G = load_i(w.control + C_GRID);
-    double mine[LANE_PARAMS];
-    int count = 0;
-    if (q.length2 > 0.0) walk_segment(w, p0, p1, layer, G, [&](double2 x) {
-      double tt;
-      if (q.accept(x, tt) && tt >= t0 && (tt < t1 || lane == 63)) {
-        // insertion into the lane's sorted run (static indices only)
-        if (count < LANE_PARAMS) {
-#pragma unroll
-          for (int r = LANE_PARAMS - 1; r >= 0; --r) {
-            bool prev_gt = r > 0 && r - 1 < count && mine[r > 0 ? r - 1 : 0] > tt;
-            double prev = mine[r > 0 ? r - 1 : 0];
-            if (r < count)
-              mine[r] = prev_gt ? prev : (mine[r] > tt ? tt : mine[r]);
-            else if (r == count)
-              mine[r] = prev_gt ? prev : tt;
-          }
+    int total = 0;
+    if (q.length2 > 0.0) {
+      const SegmentCells cells(q.a, q.b, layer, G);
+      for (int c0 = cells.cx0; c0 <= cells.cx1; c0 += 64) {
+        int cx = c0 + lane, b = 0, e = 0;
+        if (cx <= cells.cx1) {
+          int2 r = cells.run(w, cx);
+          b = r.x;
+          e = r.y;
         }
-        count += 1;
+        int rows;
+        int place = wave_prefix(max(e - b, 0), rows);
+        row0[lane] = b;
+        at[lane] = place;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        for (int f0 = 0; f0 < rows; f0 += 64) {
+          int f = f0 + lane;
+          bool ok = false;
+          double tt = 0.0;
+          if (f < rows) {
+            // the last run starting at or before f holds it (empty runs share
+            // their successor's place)
+            int r = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1)
+              if (at[r + step] <= f) r += step;
+            ok = q.accept(w.cell_xy[row0[r] + (f - at[r])], tt);
+          }
+          uint64_t m = __ballot(ok);
+          int pos = total + __popcll(m & lanemask_lt());
+          if (ok && pos < EDGE_MAX) found[pos] = tt;
+          total += __popcll(m);
+        }
+        __builtin_amdgcn_wave_barrier();  // the run table is rewritten for the next 64 columns
       }
-      return true;
-    });
-    int total;
-    int offset = wave_prefix(count, total);
+    }
     metric_max(w, M_EDGE_MAX, total);
     if (total > w.cap.edge_vertices) {
       if (lane == 0) set_error(w.status, ERR_EDGE);
       continue;
     }
-#pragma unroll
-    for (int r = 0; r < LANE_PARAMS; ++r)
-      if (r < count && count <= LANE_PARAMS) list[offset + r] = mine[r];
-    if (count > LANE_PARAMS) {
-      // a lane with a dense sub-range gathers again straight into LDS and
-      // insertion-sorts its run there
-      int m = 0;
-      walk_segment(w, p0, p1, layer, G, [&](double2 x) {
-        double tt;
-        if (q.accept(x, tt) && tt >= t0 && (tt < t1 || lane == 63)) list[offset + m++] = tt;
-        return true;
-      });
-      for (int a = 1; a < m; ++a) {
-        double v = list[offset + a];
-        int b = a - 1;
-        while (b >= 0 && list[offset + b] > v) {
-          list[offset + b + 1] = list[offset + b];
-          --b;
-        }
-        list[offset + b + 1] = v;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    // rank sort: each parameter's place is the number of parameters before it
+    // in (value, order found)
+    for (int s = lane; s < total; s += 64) {
+      double v = found[s];
+      int rank = 0;
+      for (int k = 0; k < total; ++k) {
+        double u = found[k];
+        rank += (u < v || (u == v && k < s)) ? 1 : 0;
       }
+      list[rank] = v;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
