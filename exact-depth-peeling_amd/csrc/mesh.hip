@@ -144,8 +144,6 @@ __device__ __forceinline__ int cell_index(double v, int G) {
   return c < 0.0 ? 0 : (c > (double)(G - 1) ? G - 1 : (int)c);
 }
 
-// G: power of two with G^2 about the distinct vertices per layer (copies are
-// about twice the distinct vertices), between 8 and 4096.
 // Cells per axis of each layer's grid: about two boundary vertices per cell,
 // within the cell pool.
 __device__ int grid_side(const Ws &w) {
@@ -262,19 +260,35 @@ __device__ __forceinline__ int box_cells(double2 a, double2 b, int G) {
   return cols + rows;
 }
 
-__device__ __forceinline__ void lift(const Ws &w, double2 x, double4 sc, double *out) {
-  const double *s = w.settings;
-  double u = s[0] + x.x * (s[2] - s[0]);
-  double v = s[1] + x.y * (s[3] - s[1]);
-  double z = value(sc, x);
-  if (s[4] != 0.0) {
-    z = -1.0 / z;
-    u *= z;
-    v *= z;
+// Window point -> world point: the window's range and the inverse camera are
+// read once per kernel into registers (a triangle's stores between two lifts
+// would otherwise make every lift read them again, one memory latency each).
+struct Lifter {
+  double s0, s1, s2, s3, m[12];
+  bool persp;
+  __device__ explicit Lifter(const Ws &w) {
+    const double *s = w.settings;
+    s0 = s[0];
+    s1 = s[1];
+    s2 = s[2];
+    s3 = s[3];
+    persp = s[4] != 0.0;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m[k] = w.inverse[k];
   }
-  const double *m = w.inverse;
-  for (int r = 0; r < 3; ++r) out[r] = m[4 * r] * u + m[4 * r + 1] * v + m[4 * r + 2] * z + m[4 * r + 3] * 1.0;
-}
+  __device__ void operator()(double2 x, double4 sc, double *out) const {
+    double u = s0 + x.x * (s2 - s0);
+    double v = s1 + x.y * (s3 - s1);
+    double z = value(sc, x);
+    if (persp) {
+      z = -1.0 / z;
+      u *= z;
+      v *= z;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) out[r] = m[4 * r] * u + m[4 * r + 1] * v + m[4 * r + 2] * z + m[4 * r + 3] * 1.0;
+  }
+};
 
 constexpr int EDGE_MAX = 128;
 
@@ -314,8 +328,8 @@ __device__ __forceinline__ void sort8(double (&p)[REG_PARAMS]) {
 // Fan walk over the sorted parameters: counts (emit == false) or writes
 // the triangles starting at `slot`.
 template <class Param>
-__device__ __forceinline__ int fan(const Ws &w, const EdgeQuery &q, double2 center, int count,
-                                   Param param, bool emit, int slot, int source, int layer,
+__device__ __forceinline__ int fan(const Ws &w, const Lifter &lift, const EdgeQuery &q, double2 center,
+                                   int count, Param param, bool emit, int slot, int source, int layer,
                                    double4 sc) {
   double previous = 0.0;
   double2 cursor = q.a;
@@ -334,7 +348,7 @@ __device__ __forceinline__ int fan(const Ws &w, const EdgeQuery &q, double2 cent
           for (int c = 0; c < 3; ++c) {
             uv[2 * c] = tri[c].x;
             uv[2 * c + 1] = tri[c].y;
-            lift(w, tri[c], sc, world + 3 * c);
+            lift(tri[c], sc, world + 3 * c);
           }
           w.tri_src[k] = source;
           w.tri_layer[k] = layer;
@@ -380,7 +394,9 @@ __device__ __forceinline__ bool edge_setup(const Ws &w, int j, EdgeQuery &q, dou
 // world corners.
 __device__ void mesh_edge_one(const Ws &w, int j, int n);
 
-__global__ __launch_bounds__(256) void k_mesh_edges(Ws w) {
+// Four waves per SIMD (at most 128 VGPRs): the walk is bound by memory
+// latency, so residency sets the throughput.
+__global__ __launch_bounds__(256, 4) void k_mesh_edges(Ws w) {
   // the whole block leaves or stays (mesh_edge_one synchronises the block;
   // an error bit set by another block meanwhile is seen by one read only)
   __shared__ int failed;
@@ -401,6 +417,8 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
 #pragma unroll
   for (int k = 0; k < REG_PARAMS; ++k) p[k] = 2.0;
   if (valid) valid = edge_setup(w, j, q, center, i, layer);
+  // the polygon's source is read now; its wait falls after the walk
+  const int source = valid ? w.out_src[i] : 0;
   if (valid && q.length2 > 0.0) {
     const int G = load_i(w.control + C_GRID);
     const SegmentCells cells(q.a, q.b, layer, G);
@@ -434,6 +452,8 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
     }
     sort8(p);
   }
+  // read here, after the walk has released its registers
+  const Lifter lift(w);
   int slot = wave_claim32(w.control + C_CROWDED, crowded);
   if (crowded) {
     if (slot < w.cap.vertices) w.crowded[slot] = j;
@@ -451,9 +471,8 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
     for (int r = 1; r < REG_PARAMS; ++r) v = (k == r) ? p[r] : v;
     return v;
   };
-  int source = valid ? w.out_src[i] : 0;
   double4 sc = valid ? w.scores[source] : make_double4(0, 0, 0, 0);
-  int ntri = valid ? fan(w, q, center, count, reg, false, 0, 0, 0, sc) : 0;
+  int ntri = valid ? fan(w, lift, q, center, count, reg, false, 0, 0, 0, sc) : 0;
   // one slot claim per block (returning atomics on one word serialise)
   __shared__ int wave_tris[4], block_base;
   const int wave = threadIdx.x >> 6;
@@ -471,7 +490,7 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
   int base = block_base;
   __syncthreads();  // the next edge of this block reuses the words
   if (!valid || ntri == 0) return;
-  fan(w, q, center, count, reg, true, base + offset, source, layer, sc);
+  fan(w, lift, q, center, count, reg, true, base + offset, source, layer, sc);
 }
 
 // Crowded edges, one wave each, the whole edge read by all its lanes: the
@@ -491,6 +510,7 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
   int *row0 = run_row[wave], *at = run_at[wave];
   int n = load_i(w.control + C_CROWDED);
   if (n > w.cap.vertices) n = w.cap.vertices;
+  const Lifter lift(w);
   int waves = gridDim.x * 4;
   for (int t = blockIdx.x * 4 + wave; t < n; t += waves) {
     int j = w.crowded[t];
@@ -612,7 +632,7 @@ __global__ __launch_bounds__(256) void k_mesh_crowded(Ws w) {
           for (int c = 0; c < 3; ++c) {
             uv[2 * c] = tri[c].x;
             uv[2 * c + 1] = tri[c].y;
-            lift(w, tri[c], sc, world + 3 * c);
+            lift(tri[c], sc, world + 3 * c);
           }
           w.tri_src[slot] = source;
           w.tri_layer[slot] = layer;
