@@ -7,8 +7,8 @@
 namespace peel {
 
 constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 4096 keys per block tile
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;  // 2048 keys per block tile
 constexpr int RS_MAX_SWEEPS = 8;  // 8-bit digits of a 64-bit key
 
 // Zero between sorts: allocation clears it and every sort's last kernel clears
