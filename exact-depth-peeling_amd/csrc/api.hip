@@ -243,6 +243,7 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
   c.take(w.eslots, w.ehash2_size);
   c.take(w.vcounts, R);
   c.take(w.voffsets, R);
+  c.take(w.centers, R);
   c.take(w.bverts, BV);
   c.take(w.blayers, BV);
   c.take(w.tasks, BV);

@@ -50,11 +50,16 @@ __device__ void scatter_one(const Ws &w, int i) {
   }
   int layer = w.out_layer[i];
   long R = w.cap.regions;
+  double2 center = make_double2(0.0, 0.0);
   for (int j = 0; j < n; ++j) {
-    w.bverts[base + j] = w.out_xy[j * R + i];
+    double2 v = w.out_xy[j * R + i];
+    center.x += v.x / (double)n;
+    center.y += v.y / (double)n;
+    w.bverts[base + j] = v;
     w.blayers[base + j] = layer;
-    w.tasks[base + j] = make_int2(i, j);
+    w.tasks[base + j] = make_int2(i, j | n << 16);
   }
+  w.centers[i] = center;
 }
 
 // Boundary vertices repeat once per incident polygon; only the first copy of
@@ -364,26 +369,22 @@ __device__ __forceinline__ int fan(const Ws &w, const Lifter &lift, const EdgeQu
   return ntri;
 }
 
+// Edge j of the scattered rings: its polygon's vertices are bverts[j - edge,
+// j - edge + n), and k_scatter left the polygon's centroid (summed in ring
+// order) in centers.
 __device__ __forceinline__ bool edge_setup(const Ws &w, int j, EdgeQuery &q, double2 &center, int &i,
                                            int &layer) {
   int2 task = w.tasks[j];
   i = task.x;
-  int edge = task.y;
-  int n = w.out_n[i];
+  const int edge = task.y & 0xFFFF, n = task.y >> 16;
   if (edge >= n) return false;
-  long R = w.cap.regions;
-  q.a = w.out_xy[edge * R + i];
-  q.b = w.out_xy[((edge + 1) % n) * R + i];
+  q.a = w.bverts[j];
+  q.b = w.bverts[edge + 1 < n ? j + 1 : j - edge];
   q.d = make_double2(q.b.x - q.a.x, q.b.y - q.a.y);
   q.length2 = q.d.x * q.d.x + q.d.y * q.d.y;
   q.len = sqrt(q.length2);
-  center = make_double2(0.0, 0.0);
-  for (int k = 0; k < n; ++k) {
-    double2 v = w.out_xy[k * R + i];
-    center.x += v.x / (double)n;
-    center.y += v.y / (double)n;
-  }
-  layer = w.out_layer[i];
+  center = w.centers[i];
+  layer = w.blayers[j];
   return true;
 }
 

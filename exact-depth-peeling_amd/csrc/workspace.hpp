@@ -130,9 +130,10 @@ struct Ws {
 
   // meshing
   int *vcounts, *voffsets;  // [regions]
+  double2 *centers;         // [regions] polygon centroids
   double2 *bverts;          // [vertices]
   int *blayers;             // [vertices]
-  int2 *tasks;              // [vertices]
+  int2 *tasks;              // [vertices] (polygon, edge | vertex count << 16)
   int *crowded;             // [vertices] edges for the per-lane-array noding path
   int *phash;               // [phash_size] boundary vertex dedupe (layer, x, y)
   // per-layer cell grid of the distinct boundary vertices, compressed rows:
