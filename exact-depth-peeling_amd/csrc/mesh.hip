@@ -25,10 +25,11 @@ __device__ __forceinline__ int npolys_m(const Ws &w) {
 __device__ void scatter_one(const Ws &w, int i);
 
 // After the exclusive scan of the polygons' vertex counts (their ring
-// offsets): check and publish the boundary-vertex total, clear the live part
-// of the point table for k_point_dedupe, and scatter every ring's vertices.
-__global__ void k_scatter(Ws w) {
-  int total = load_i(w.control + C_NBV);
+// offsets, total in control word `total_word`): check and publish the
+// boundary-vertex total, clear the live part of the point table for
+// k_point_dedupe, and scatter every ring's vertices.
+__global__ void k_scatter(Ws w, int total_word) {
+  int total = load_i(w.control + total_word);
   int live = total > w.cap.vertices ? 0 : total;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     if (total > w.cap.vertices) set_error(w.status, ERR_VERTS);
@@ -667,9 +668,14 @@ __global__ void k_metrics_fold(Ws w) {
 }
 
 void enqueue_mesh(const Ws &w, hipStream_t s, Timeline *t) {
-  // ring offsets: an exclusive scan of the polygons' vertex counts
-  scan_exclusive(s, w.out_n, w.voffsets, w.status + ST_POLYS, 0, w.cap.regions, w.control + C_NBV, w.scan);
-  hipLaunchKernelGGL(k_scatter, stride_grid(w.cap.regions), dim3(256), 0, s, w);
+  // ring offsets: an exclusive scan of the polygons' vertex counts. The
+  // connectivity stage scanned the same counts into the same offsets; with
+  // every layer kept (no ring is emptied by the layer limit) and no merge,
+  // the rings are unchanged since, and its offsets and total serve here.
+  const bool scanned = w.route != ROUTE_LAYERED && w.propagate_ranks && !w.merge_regions && w.all_layers;
+  if (!scanned)
+    scan_exclusive(s, w.out_n, w.voffsets, w.status + ST_POLYS, 0, w.cap.regions, w.control + C_NBV, w.scan);
+  hipLaunchKernelGGL(k_scatter, stride_grid(w.cap.regions), dim3(256), 0, s, w, scanned ? (int)C_PEDGES : (int)C_NBV);
   hipLaunchKernelGGL(k_point_dedupe, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
   mark(t, s, "scatter_vertices");
   hipLaunchKernelGGL(k_cell_count, stride_grid(w.cap.vertices), dim3(256), 0, s, w);
