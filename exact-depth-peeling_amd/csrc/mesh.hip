@@ -313,7 +313,6 @@ struct EdgeQuery {
 };
 
 constexpr int REG_PARAMS = 8;
-constexpr int DENSE_ROWS = 128;
 
 // Sorting network on 8 registers (Batcher odd-even merge), static indices only.
 __device__ __forceinline__ void cswap(double &x, double &y) {
@@ -424,17 +423,8 @@ __device__ void mesh_edge_one(const Ws &w, int j, int n) {
   if (valid && q.length2 > 0.0) {
     const int G = load_i(w.control + C_GRID);
     const SegmentCells cells(q.a, q.b, layer, G);
-    // long edges, and edges through more than DENSE_ROWS vertices, go
-    // straight to the wave-parallel path
+    // long edges go straight to the wave-parallel path
     crowded = box_cells(q.a, q.b, G) > w.long_edge_cells;
-    if (!crowded) {
-      int rows = 0;
-      for (int cx = cells.cx0; cx <= cells.cx1; ++cx) {
-        int2 r = cells.run(w, cx);
-        rows += max(r.y - r.x, 0);
-      }
-      crowded = rows > DENSE_ROWS;
-    }
     auto visit = [&](double2 x) {
       double t;
       if (q.accept(x, t)) {
