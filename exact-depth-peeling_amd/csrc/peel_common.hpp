@@ -112,6 +112,7 @@ enum Control : int {
   C_RAYSORT = 145,   // leaders to put in Morton order (0 when a wave walks each ray)
   C_HUGE_SYNC = 160, // arrivals at the grid barriers of the huge-group sort
   C_RAYKEY_DONE = 192, // blocks of k_ray_keys that have finished
+  C_RAYFALL = 224,   // rays k_ray_bfs leaves to k_ray_hits (their level lists outgrew its LDS)
   C_COUNT = 256,
 };
 

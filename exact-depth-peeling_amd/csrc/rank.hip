@@ -615,88 +615,205 @@ __global__ __launch_bounds__(256) void k_ray_keys(Ws w) {
   }
 }
 
+// Counters of a ray stage, summed per wave at the end of the kernel.
+struct RayCounts {
+  long long groups = 0, visits = 0, heavy = 0;
+  int filtered = 0, fallbacks = 0;
+};
+
+// The rays order[base + lane / K] (per wave 64 / K of them), K lanes each:
+// every lane walks its subtree at depth log2 K and hands the rest of it to
+// k_heavy_hits after HANDOFF candidates. Called by every lane of the wave
+// together.
+__device__ void walk_ray_lanes(const Ws &w, HitStage &stage, const Tree &tree, int K, long base, int n,
+                               const int *order, RayCounts &rc) {
+  const int lane = threadIdx.x & 63, sub = lane & (K - 1);
+  long r = base + lane / K;
+  bool active = r < n;
+  int i = active ? order[r] : 0;
+  double2 x = active ? w.samples[i] : make_double2(0, 0);
+  float2 x32 = make_float2((float)x.x, (float)x.y);
+  float4 lo, hi;
+  ray_box(x, lo, hi);
+  const int root = K + sub;
+  int node = active ? root : 0;  // binary continuation, for hand-offs
+  const int end = successor(root);
+  TrailWalk walk;
+  walk.trail = 0;
+  walk.bucket = 0;
+  if (active) walk.start(tree, root, w.bucket_log, lo, hi);
+  int count = 0, visits = 0;
+  bool handed = false, queue_full = false;
+  bool live = node > 0 && node != end;
+  while (__ballot(live)) {  // wave-uniform: the hit stage needs every lane
+    bool hit = false;
+    int leaf = 0;
+    if (live && visits >= HANDOFF && !queue_full) {
+      // the unvisited suffix of this subtree starts at the first leaf
+      // below the cursor
+      int first = node;
+      while (first < tree.leaves) first *= 2;
+      int slot = wave_claim32(w.control + C_HEAVY, true);
+      if (slot < w.cap.regions) {
+        w.heavy[slot] = make_int4(i, first, root, 0);
+        handed = true;
+        live = false;
+      } else {
+        queue_full = true;
+      }
+    }
+    if (live) {
+      int leaf_node = walk.next_node(tree, lo, hi);
+      if (leaf_node < 0) {
+        live = false;
+      } else {
+        node = successor(leaf_node);
+        leaf = leaf_node - tree.leaves;
+        visits += 1;
+        hit = contains_any(w, leaf, x, x32, rc.filtered, rc.fallbacks);
+        count += hit;
+        live = node > 0 && node != end;
+      }
+    }
+    stage_hits(w, stage, hit, leaf, i, x);
+  }
+  long long total = group_sum(count, K);
+  long long tested = group_sum(visits, K);
+  long long any_handed = group_sum(handed ? 1 : 0, K);
+  if (active && sub == 0) {
+    w.group_counts[i] = (int)total;
+    rc.groups += 1;
+    rc.visits += tested;
+    rc.heavy += (any_handed > 0 || tested > 32) ? 1 : 0;
+  }
+}
+
+__device__ void add_ray_counts(const Ws &w, const RayCounts &rc) {
+  metric_add(w, M_RAY_GROUPS, rc.groups);
+  metric_add(w, M_RAY_TESTS, rc.visits);
+  metric_add(w, M_HEAVY, rc.heavy);
+  if (w.filter32) {
+    metric_add(w, M_FILTER_TESTS, rc.filtered);
+    metric_add(w, M_FALLBACKS, rc.fallbacks);
+  }
+}
+
 // ray_hits (event_kernels.py:196-245), leaders only: the first C_LEADERS
-// entries of the Morton-sorted ray order, K lanes each.
+// entries of the Morton-sorted ray order, K lanes each. With a whole wave per
+// ray (K = 64) k_ray_bfs, launched before, has taken the rays, and this launch
+// walks only those it listed (their level lists outgrew its LDS).
 __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
   if (w.status[ST_ERR] != 0) return;
-  HitStage stage = hit_stage();
-  const int n = load_i(w.control + C_LEADERS);
-  const int lane = threadIdx.x & 63;
+  const int leaders = load_i(w.control + C_LEADERS);
   const Tree tree = w.source_tree.tree();
-  const int K = ray_lanes(tree, n, w.ray_lanes_target);
-  const int sub = lane & (K - 1);
+  const int K = ray_lanes(tree, leaders, w.ray_lanes_target);
+  const bool listed = K == 64;
+  const int n = listed ? load_i(w.control + C_RAYFALL) : leaders;
+  const int *order = listed ? w.ray_ids_alt : w.ray_ids;
+  HitStage stage = hit_stage();
   const int per_wave = 64 / K;
   const long waves = (gridDim.x * blockDim.x) >> 6;
-  long long groups = 0, visits_all = 0, heavy = 0;
-  int filtered = 0, fallbacks = 0;
-  for (long base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * per_wave; base < n; base += waves * per_wave) {
-    long r = base + lane / K;
-    bool active = r < n;
-    int i = active ? w.ray_ids[r] : 0;
-    double2 x = active ? w.samples[i] : make_double2(0, 0);
-    float2 x32 = make_float2((float)x.x, (float)x.y);
+  RayCounts rc;
+  for (long base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * per_wave; base < n; base += waves * per_wave)
+    walk_ray_lanes(w, stage, tree, K, base, n, order, rc);
+  flush_hits(w, stage);
+  add_ray_counts(w, rc);
+}
+
+// The same rays, one wave per ray, breadth first: each level of the source
+// hierarchy's four-wide walk is expanded by the whole wave at once (a lane per
+// grandchild box, the overlapping ones appended to the next level's list in
+// LDS), then the lanes test the leaves of the overlapping buckets 64 at a
+// time. A ray costs one dependent read per level plus one per 64 leaf boxes,
+// where a lane walking its own subtree reads once per step of its longest
+// walk. The candidates are the same leaves (those whose boxes meet the ray
+// box), so the hits and counts are too; the hit order is restored by the sort.
+// A ray whose list outgrows RAY_FRONTIER is listed (its patch index, in
+// ray_ids_alt, which the skipped Morton sort leaves free) for k_ray_hits.
+constexpr int RAY_FRONTIER = 512;
+constexpr int RAY_BFS_BLOCKS = 2048;  // 8,192 waves: the most rays that take a wave each
+
+__global__ __launch_bounds__(256) void k_ray_bfs(Ws w) {
+  __shared__ int frontier_lds[4][2][RAY_FRONTIER];
+  if (w.status[ST_ERR] != 0) return;
+  const int n = load_i(w.control + C_LEADERS);
+  const Tree tree = w.source_tree.tree();
+  if (ray_lanes(tree, n, w.ray_lanes_target) != 64) return;
+  HitStage stage = hit_stage();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t below = lanemask_lt();
+  TrailWalk plan;
+  plan.plan(tree, 1, w.bucket_log);
+  const int lvls = plan.lvls, blog = plan.blog;
+  const long waves = (gridDim.x * blockDim.x) >> 6;
+  RayCounts rc;
+  for (long r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < n; r += waves) {
+    const int i = w.ray_ids[r];
+    const double2 x = w.samples[i];
+    const float2 x32 = make_float2((float)x.x, (float)x.y);
     float4 lo, hi;
     ray_box(x, lo, hi);
-    const int root = K + sub;
-    int node = active ? root : 0;  // binary continuation, for hand-offs
-    const int end = successor(root);
-    TrailWalk walk;
-    walk.trail = 0;
-    walk.bucket = 0;
-    if (active) walk.start(tree, root, w.bucket_log, lo, hi);
-    int count = 0, visits = 0;
-    bool handed = false, queue_full = false;
-    bool live = node > 0 && node != end;
-    while (__ballot(live)) {  // wave-uniform: the hit stage needs every lane
+    int *cur = frontier_lds[wave][0], *next = frontier_lds[wave][1];
+    if (lane == 0) cur[0] = 1;
+    int count = 1;  // wave-uniform
+    bool overflow = false;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    for (int k = 0; k < lvls && count > 0 && !overflow; ++k) {
+      int made = 0;
+      for (int c0 = 0; c0 < count; c0 += 16) {
+        const int idx = c0 + (lane >> 2);
+        bool keep = false;
+        int child = 0;
+        if (idx < count) {
+          child = 4 * cur[idx] + (lane & 3);
+          keep = overlaps(tree.nodes[child], lo, hi);
+        }
+        const uint64_t m = __ballot(keep);
+        const int pos = made + __popcll(m & below);
+        if (keep && pos < RAY_FRONTIER) next[pos] = child;
+        made += __popcll(m);
+      }
+      overflow = made > RAY_FRONTIER;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      int *t = cur;
+      cur = next;
+      next = t;
+      count = made;
+    }
+    if (overflow) {
+      if (lane == 0) w.ray_ids_alt[atomicAdd(w.control + C_RAYFALL, 1)] = i;
+      continue;
+    }
+    // the buckets' leaves, 64 (bucket, leaf) pairs at a time
+    int hits = 0, visits = 0;
+    const int pairs = count << blog;
+    for (int q0 = 0; q0 < pairs; q0 += 64) {
+      const int q = q0 + lane;
       bool hit = false;
       int leaf = 0;
-      if (live && visits >= HANDOFF && !queue_full) {
-        // the unvisited suffix of this subtree starts at the first leaf
-        // below the cursor
-        int first = node;
-        while (first < tree.leaves) first *= 2;
-        int slot = wave_claim32(w.control + C_HEAVY, true);
-        if (slot < w.cap.regions) {
-          w.heavy[slot] = make_int4(i, first, root, 0);
-          handed = true;
-          live = false;
-        } else {
-          queue_full = true;
-        }
-      }
-      if (live) {
-        int leaf_node = walk.next_node(tree, lo, hi);
-        if (leaf_node < 0) {
-          live = false;
-        } else {
-          node = successor(leaf_node);
+      if (q < pairs) {
+        const int leaf_node = (cur[q >> blog] << blog) + (q & ((1 << blog) - 1));
+        if (overlaps(tree.nodes[leaf_node], lo, hi)) {
           leaf = leaf_node - tree.leaves;
           visits += 1;
-          hit = contains_any(w, leaf, x, x32, filtered, fallbacks);
-          count += hit;
-          live = node > 0 && node != end;
+          hit = contains_any(w, leaf, x, x32, rc.filtered, rc.fallbacks);
+          hits += hit;
         }
       }
       stage_hits(w, stage, hit, leaf, i, x);
     }
-    long long total = group_sum(count, K);
-    long long tested = group_sum(visits, K);
-    long long any_handed = group_sum(handed ? 1 : 0, K);
-    if (active && sub == 0) {
+    const long long total = wave_sum(hits), tested = wave_sum(visits);
+    if (lane == 0) {
       w.group_counts[i] = (int)total;
-      groups += 1;
-      visits_all += tested;
-      heavy += (any_handed > 0 || tested > 32) ? 1 : 0;
+      rc.groups += 1;
+      rc.visits += tested;
+      rc.heavy += tested > 32 ? 1 : 0;
     }
   }
   flush_hits(w, stage);
-  metric_add(w, M_RAY_GROUPS, groups);
-  metric_add(w, M_RAY_TESTS, visits_all);
-  metric_add(w, M_HEAVY, heavy);
-  if (w.filter32) {
-    metric_add(w, M_FILTER_TESTS, filtered);
-    metric_add(w, M_FALLBACKS, fallbacks);
-  }
+  add_ray_counts(w, rc);
 }
 
 // heavy_ray_hits (event_kernels.py:248-298): each handed-off job (ray, first
@@ -1253,6 +1370,8 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   radix_sort(s, w.ray_keys, w.ray_ids, w.ray_keys_alt, w.ray_ids_alt, w.control + C_RAYSORT, 0,
              w.cap.regions, 0, 32, w.sort);
   mark(t, s, "samples_groups");
+  hipLaunchKernelGGL(k_ray_bfs, dim3(std::min(stride_grid(w.cap.regions).x, (unsigned)RAY_BFS_BLOCKS)), dim3(256), 0,
+                     s, w);
   hipLaunchKernelGGL(k_ray_hits, stride_grid(w.cap.regions), dim3(256), 0, s, w);
   mark(t, s, "k_ray_hits");
   {
