@@ -108,7 +108,7 @@ int peel::plan(Ws &w, const PeelDesc *d) {
   w.sort.max_blocks = (int)((sort_cap + RS_TILE - 1) / RS_TILE);
   long scan_cap = std::max<long>({(long)w.cap.regions, BV, (long)w.nseg, w.cell_capacity});
   w.scan.max_blocks = (int)((scan_cap + 4095) / 4096) + 1;
-  w.ray_lanes_target = 524288;
+  w.ray_lanes_target = 4194304;
   w.heavy_lanes = 65536;
   w.long_edge_cells = 48;
   w.bucket_log = 2;

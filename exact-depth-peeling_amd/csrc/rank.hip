@@ -542,11 +542,22 @@ __device__ __forceinline__ void ray_box(double2 x, float4 &lo, float4 &hi) {
 // meaning.
 constexpr int HANDOFF = 32;
 
-// Lanes per ray: a whole wave (one leaf subtree per lane) while 64 lanes per
-// ray stay within `target` lanes in flight (524,288 by default: enough to fill
-// 256 CUs), else one.
-__device__ __forceinline__ int ray_lanes(const Tree &tree, int rays, int target) {
-  if ((long)64 * rays > (long)target) return 1;
+// Lanes per ray. A ray walked breadth first (k_ray_bfs) takes a group of 64
+// lanes while 64 lanes per ray stay within `target` lanes in flight (4,194,304
+// by default), else a group of 16 while 16 do; past that a ray takes one lane
+// and is walked depth first (k_ray_hits, rays in Morton order). On the
+// 717,740-face shells (79,100 rays) groups of 16 took 0.40 ms, groups of 64
+// 0.46 ms and one lane per ray 0.80 ms; the 65,536-face torus knot (4,124
+// rays) takes 0.084 ms in groups of 64.
+__device__ __forceinline__ int ray_group(int rays, int target) {
+  if ((long)64 * rays <= (long)target) return 64;
+  if ((long)16 * rays <= (long)target) return 16;
+  return 1;
+}
+// Lanes of a depth-first ray walk (one leaf subtree each): one per ray when the
+// rays take a lane each; else a whole wave, for the rays k_ray_bfs hands over.
+__device__ __forceinline__ int walk_lanes(const Tree &tree, int group) {
+  if (group == 1) return 1;
   return tree.leaves < 64 ? tree.leaves : 64;
 }
 
@@ -610,7 +621,7 @@ __global__ __launch_bounds__(256) void k_ray_keys(Ws w) {
   __syncthreads();
   if (last && threadIdx.x == 0) {
     int leaders = load_i(w.control + C_LEADERS);
-    w.control[C_RAYSORT] = ray_lanes(w.source_tree.tree(), leaders, w.ray_lanes_target) == 1 ? leaders : 0;
+    w.control[C_RAYSORT] = ray_group(leaders, w.ray_lanes_target) == 1 ? leaders : 0;
     w.control[C_RAYKEY_DONE] = 0;
   }
 }
@@ -699,15 +710,15 @@ __device__ void add_ray_counts(const Ws &w, const RayCounts &rc) {
 }
 
 // ray_hits (event_kernels.py:196-245), leaders only: the first C_LEADERS
-// entries of the Morton-sorted ray order, K lanes each. With a whole wave per
-// ray (K = 64) k_ray_bfs, launched before, has taken the rays, and this launch
-// walks only those it listed (their level lists outgrew its LDS).
+// entries of the Morton-sorted ray order, one lane each. When the rays take
+// groups of lanes, k_ray_bfs, launched before, has taken them, and this launch
+// walks only those it listed (their level lists outgrew its LDS), a wave each.
 __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
   if (w.status[ST_ERR] != 0) return;
   const int leaders = load_i(w.control + C_LEADERS);
   const Tree tree = w.source_tree.tree();
-  const int K = ray_lanes(tree, leaders, w.ray_lanes_target);
-  const bool listed = K == 64;
+  const int G = ray_group(leaders, w.ray_lanes_target), K = walk_lanes(tree, G);
+  const bool listed = G > 1;
   const int n = listed ? load_i(w.control + C_RAYFALL) : leaders;
   const int *order = listed ? w.ray_ids_alt : w.ray_ids;
   HitStage stage = hit_stage();
@@ -720,77 +731,81 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
   add_ray_counts(w, rc);
 }
 
-// The same rays, one wave per ray, breadth first: each level of the source
-// hierarchy's four-wide walk is expanded by the whole wave at once (a lane per
-// grandchild box, the overlapping ones appended to the next level's list in
-// LDS), then the lanes test the leaves of the overlapping buckets 64 at a
-// time. A ray costs one dependent read per level plus one per 64 leaf boxes,
+// The same rays breadth first, a group of G lanes (64 or 16) each: each level
+// of the source hierarchy's four-wide walk is expanded by the group at once (a
+// lane per grandchild box, the overlapping ones appended to the next level's
+// list in LDS), then the group's lanes test the leaves of the overlapping
+// buckets. A ray costs one dependent read per level plus one per G leaf boxes,
 // where a lane walking its own subtree reads once per step of its longest
 // walk. The candidates are the same leaves (those whose boxes meet the ray
 // box), so the hits and counts are too; the hit order is restored by the sort.
-// A ray whose list outgrows RAY_FRONTIER is listed (its patch index, in
-// ray_ids_alt, which the skipped Morton sort leaves free) for k_ray_hits.
+// A ray whose list outgrows its share of RAY_FRONTIER is listed (its patch
+// index, in ray_ids_alt, which the skipped Morton sort leaves free) for
+// k_ray_hits. Loops run while any group of the wave has work (the ballots and
+// the hit stage need every lane).
 constexpr int RAY_FRONTIER = 512;
-constexpr int RAY_BFS_BLOCKS = 2048;  // 8,192 waves: the most rays that take a wave each
+constexpr int RAY_BFS_BLOCKS = 2048;  // 8,192 waves
 
 __global__ __launch_bounds__(256) void k_ray_bfs(Ws w) {
   __shared__ int frontier_lds[4][2][RAY_FRONTIER];
   if (w.status[ST_ERR] != 0) return;
   const int n = load_i(w.control + C_LEADERS);
   const Tree tree = w.source_tree.tree();
-  if (ray_lanes(tree, n, w.ray_lanes_target) != 64) return;
+  const int G = ray_group(n, w.ray_lanes_target);
+  if (G == 1) return;
   HitStage stage = hit_stage();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint64_t below = lanemask_lt();
+  const int per_wave = 64 / G, g = lane / G, gl = lane % G, cap = RAY_FRONTIER / per_wave;
+  const uint64_t gmask = G == 64 ? ~0ull : 0xFFFFull << (16 * g);
+  const uint64_t below = lanemask_lt() & gmask;
   TrailWalk plan;
   plan.plan(tree, 1, w.bucket_log);
   const int lvls = plan.lvls, blog = plan.blog;
   const long waves = (gridDim.x * blockDim.x) >> 6;
   RayCounts rc;
-  for (long r = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; r < n; r += waves) {
-    const int i = w.ray_ids[r];
-    const double2 x = w.samples[i];
+  for (long r0 = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * per_wave; r0 < n; r0 += waves * per_wave) {
+    const long r = r0 + g;
+    const bool active = r < n;  // group-uniform, as every value below
+    const int i = active ? w.ray_ids[r] : 0;
+    const double2 x = active ? w.samples[i] : make_double2(0, 0);
     const float2 x32 = make_float2((float)x.x, (float)x.y);
     float4 lo, hi;
     ray_box(x, lo, hi);
-    int *cur = frontier_lds[wave][0], *next = frontier_lds[wave][1];
-    if (lane == 0) cur[0] = 1;
-    int count = 1;  // wave-uniform
+    int sel = 0;
+    if (gl == 0) frontier_lds[wave][0][g * cap] = 1;
+    int count = active ? 1 : 0;
     bool overflow = false;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
-    for (int k = 0; k < lvls && count > 0 && !overflow; ++k) {
+    for (int k = 0; k < lvls; ++k) {
+      const int *cur = &frontier_lds[wave][sel][g * cap];
+      int *next = &frontier_lds[wave][sel ^ 1][g * cap];
       int made = 0;
-      for (int c0 = 0; c0 < count; c0 += 16) {
-        const int idx = c0 + (lane >> 2);
+      for (int c0 = 0; __ballot(c0 < count && !overflow); c0 += G / 4) {
+        const int idx = c0 + (gl >> 2);
         bool keep = false;
         int child = 0;
-        if (idx < count) {
+        if (idx < count && !overflow) {
           child = 4 * cur[idx] + (lane & 3);
           keep = overlaps(tree.nodes[child], lo, hi);
         }
-        const uint64_t m = __ballot(keep);
+        const uint64_t m = __ballot(keep) & gmask;
         const int pos = made + __popcll(m & below);
-        if (keep && pos < RAY_FRONTIER) next[pos] = child;
+        if (keep && pos < cap) next[pos] = child;
         made += __popcll(m);
       }
-      overflow = made > RAY_FRONTIER;
+      overflow = overflow || made > cap;
+      count = made;
+      sel ^= 1;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
-      int *t = cur;
-      cur = next;
-      next = t;
-      count = made;
     }
-    if (overflow) {
-      if (lane == 0) w.ray_ids_alt[atomicAdd(w.control + C_RAYFALL, 1)] = i;
-      continue;
-    }
-    // the buckets' leaves, 64 (bucket, leaf) pairs at a time
+    // the buckets' leaves, G (bucket, leaf) pairs at a time
+    const int *cur = &frontier_lds[wave][sel][g * cap];
+    const int pairs = overflow ? 0 : count << blog;
     int hits = 0, visits = 0;
-    const int pairs = count << blog;
-    for (int q0 = 0; q0 < pairs; q0 += 64) {
-      const int q = q0 + lane;
+    for (int q0 = 0; __ballot(q0 < pairs); q0 += G) {
+      const int q = q0 + gl;
       bool hit = false;
       int leaf = 0;
       if (q < pairs) {
@@ -804,12 +819,16 @@ __global__ __launch_bounds__(256) void k_ray_bfs(Ws w) {
       }
       stage_hits(w, stage, hit, leaf, i, x);
     }
-    const long long total = wave_sum(hits), tested = wave_sum(visits);
-    if (lane == 0) {
-      w.group_counts[i] = (int)total;
-      rc.groups += 1;
-      rc.visits += tested;
-      rc.heavy += tested > 32 ? 1 : 0;
+    const long long total = group_sum(hits, G), tested = group_sum(visits, G);
+    if (active && gl == 0) {
+      if (overflow) {
+        w.ray_ids_alt[atomicAdd(w.control + C_RAYFALL, 1)] = i;
+      } else {
+        w.group_counts[i] = (int)total;
+        rc.groups += 1;
+        rc.visits += tested;
+        rc.heavy += tested > 32 ? 1 : 0;
+      }
     }
   }
   flush_hits(w, stage);
@@ -827,7 +846,7 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   int jobs = ok ? load_i(w.control + C_HEAVY) : 0;
   if (jobs > w.cap.regions) jobs = w.cap.regions;
   // leaves per lane subtree, with the lanes per ray that k_ray_hits chose
-  const int W = tree.leaves / ray_lanes(tree, load_i(w.control + C_LEADERS), w.ray_lanes_target);
+  const int W = tree.leaves / walk_lanes(tree, ray_group(load_i(w.control + C_LEADERS), w.ray_lanes_target));
   // a job split into more parts re-enters that many subtrees from their
   // roots: split only while the split jobs still fit the heavy lane budget
   int parts = W;

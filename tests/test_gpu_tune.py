@@ -19,7 +19,7 @@ def test_tuning_arguments():
     p = knot_peeler()
     t = p.tuning
     assert t["persistent_blocks_per_cu"] == 1 and t["max_blocks_per_cu"] >= 1
-    assert t["ray_lanes_target"] == 524288 and t["heavy_lanes"] == 65536 and t["long_edge_cells"] == 48
+    assert t["ray_lanes_target"] == 4194304 and t["heavy_lanes"] == 65536 and t["long_edge_cells"] == 48
     assert t["leaf_bucket_log2"] == 2
     for bad in ({"persistent_blocks_per_cu": 0}, {"persistent_blocks_per_cu": t["max_blocks_per_cu"] + 1},
                 {"ray_lanes_target": 8}, {"heavy_lanes": 10}, {"long_edge_cells": 0}, {"leaf_bucket_log2": 5},
@@ -47,6 +47,25 @@ def test_outputs_do_not_depend_on_tuning(filter_fp32):
         p.run()
         _, r, s, l, uv, w, ts, tl = gpu_outputs(p)
         assert (canonical_patches(r, s, l), canonical_triangles(uv, w, ts, tl)) == want, setting
+
+
+@pytest.mark.parametrize("filter_fp32", [False, True])
+def test_ray_walks_agree(filter_fp32):
+    """Rays in groups of 64 lanes and of 16 (breadth first, k_ray_bfs) and one
+    lane per ray (depth first, k_ray_hits) give the same patches, triangles
+    and candidate-test totals."""
+    print(f"[tune] ray walk modes fp32={filter_fp32}", flush=True)
+    p = knot_peeler(filter_fp32=filter_fp32)
+    p.run()
+    rays = p.read_status()["ray_groups"]
+    assert rays > 8
+    results = []
+    for target in (1 << 30, 24 * rays, 64):  # groups of 64, of 16, one lane each
+        p.set_tuning(ray_lanes_target=target)
+        p.run()
+        info, r, s, l, uv, w, ts, tl = gpu_outputs(p)
+        results.append((canonical_patches(r, s, l), canonical_triangles(uv, w, ts, tl)))
+    assert results[0] == results[1] == results[2]
 
 
 def test_autotune_and_cache(tmp_path):
