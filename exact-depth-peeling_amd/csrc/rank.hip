@@ -542,16 +542,19 @@ __device__ __forceinline__ void ray_box(double2 x, float4 &lo, float4 &hi) {
 // meaning.
 constexpr int HANDOFF = 32;
 
-// Lanes per ray. A ray walked breadth first (k_ray_bfs) takes a group of 64
-// lanes while 64 lanes per ray stay within `target` lanes in flight (4,194,304
-// by default), else a group of 16 while 16 do; past that a ray takes one lane
-// and is walked depth first (k_ray_hits, rays in Morton order). On the
-// 717,740-face shells (79,100 rays) groups of 16 took 0.40 ms, groups of 64
-// 0.46 ms and one lane per ray 0.80 ms; the 65,536-face torus knot (4,124
-// rays) takes 0.084 ms in groups of 64.
+// Lanes per ray. A ray walked breadth first (k_ray_bfs) takes a group of 16
+// lanes while 16 lanes per ray stay within `target` lanes in flight (4,194,304
+// by default), or a whole wave when there are at most RAY_FEW rays; past the
+// target a ray takes one lane and is walked depth first (k_ray_hits, rays in
+// Morton order). Groups of 16 measured best: the 717,740-face shells (79,100
+// rays) 0.39 ms against 0.46 in groups of 64, 0.39 in groups of 8 and 0.80
+// with a lane per ray; the 65,536-face torus knot (4,124 rays) 0.048 ms
+// against 0.075 in groups of 64 and 0.071 in groups of 8.
+constexpr int RAY_FEW = 2048;  // rays at most that take 64 lanes each
+constexpr int RAY_GROUP = 16;  // lanes of the other breadth-first rays
 __device__ __forceinline__ int ray_group(int rays, int target) {
-  if ((long)64 * rays <= (long)target) return 64;
-  if ((long)16 * rays <= (long)target) return 16;
+  if (rays <= RAY_FEW && (long)64 * rays <= (long)target) return 64;
+  if ((long)RAY_GROUP * rays <= (long)target) return RAY_GROUP;
   return 1;
 }
 // Lanes of a depth-first ray walk (one leaf subtree each): one per ray when the
@@ -756,7 +759,7 @@ __global__ __launch_bounds__(256) void k_ray_bfs(Ws w) {
   HitStage stage = hit_stage();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int per_wave = 64 / G, g = lane / G, gl = lane % G, cap = RAY_FRONTIER / per_wave;
-  const uint64_t gmask = G == 64 ? ~0ull : 0xFFFFull << (16 * g);
+  const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1) << (G * g);
   const uint64_t below = lanemask_lt() & gmask;
   TrailWalk plan;
   plan.plan(tree, 1, w.bucket_log);
