@@ -212,8 +212,6 @@ void peel::carve(Ws &w, Carver &c, bool outputs) {
   c.take(w.group_counts, R);
   c.take(w.group_offsets, R);
   c.take(w.group_fill, R);
-  c.take(w.ray_keys, R);
-  c.take(w.ray_keys_alt, R);
   c.take(w.ray_ids, R);
   c.take(w.ray_ids_alt, R);
   c.take(w.medium, R);

@@ -109,9 +109,7 @@ enum Control : int {
   C_CELLS = 146,     // live cells of that grid (layers * G^2)
   C_LIVE = 143,      // patches left after convex coalescing
   C_SORTN = 144,     // layered route: residual pieces to order (0 once the loop ends)
-  C_RAYSORT = 145,   // leaders to put in Morton order (0 when a wave walks each ray)
   C_HUGE_SYNC = 160, // arrivals at the grid barriers of the huge-group sort
-  C_RAYKEY_DONE = 192, // blocks of k_ray_keys that have finished
   C_RAYFALL = 224,   // rays k_ray_bfs leaves to k_ray_hits (their level lists outgrew its LDS)
   C_COUNT = 256,
 };
@@ -481,19 +479,6 @@ __device__ __forceinline__ int wave_prefix(int v, int &total) {
 __device__ __forceinline__ void wave_add_metric(long long *metric, long long v) {
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long *)metric, (unsigned long long)v);
-}
-
-// 30-bit Morton code of a point clamped to [0, 1]^2 (x in even bits).
-__device__ __forceinline__ uint64_t morton2(float cx, float cy) {
-  cx = fminf(fmaxf(cx, 0.0f), 1.0f);
-  cy = fminf(fmaxf(cy, 0.0f), 1.0f);
-  uint32_t x = (uint32_t)(int)(cx * 32767.0f), y = (uint32_t)(int)(cy * 32767.0f);
-  uint64_t key = 0;
-  for (int bit = 0; bit < 15; ++bit) {
-    key |= (uint64_t)((x >> bit) & 1u) << (2 * bit);
-    key |= (uint64_t)((y >> bit) & 1u) << (2 * bit + 1);
-  }
-  return key;
 }
 
 // Same traversal, returning the leaf index (position in Morton order).

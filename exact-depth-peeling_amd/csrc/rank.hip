@@ -542,27 +542,23 @@ __device__ __forceinline__ void ray_box(double2 x, float4 &lo, float4 &hi) {
 // meaning.
 constexpr int HANDOFF = 32;
 
-// Lanes per ray. A ray walked breadth first (k_ray_bfs) takes a group of 16
-// lanes while 16 lanes per ray stay within `target` lanes in flight (4,194,304
-// by default), or a whole wave when there are at most RAY_FEW rays; past the
-// target a ray takes one lane and is walked depth first (k_ray_hits, rays in
-// Morton order). Groups of 16 measured best: the 717,740-face shells (79,100
-// rays) 0.39 ms against 0.46 in groups of 64, 0.39 in groups of 8 and 0.80
-// with a lane per ray; the 65,536-face torus knot (4,124 rays) 0.048 ms
-// against 0.075 in groups of 64 and 0.071 in groups of 8.
-constexpr int RAY_FEW = 2048;  // rays at most that take 64 lanes each
-constexpr int RAY_GROUP = 16;  // lanes of the other breadth-first rays
+// Lanes per ray of the breadth-first walk (k_ray_bfs): 16, or a whole wave
+// when there are at most RAY_FEW rays and 64 lanes per ray stay within
+// `target` lanes in flight. Groups of 16 measured best: the 717,740-face
+// shells (79,100 rays) 0.38 ms against 0.46 in groups of 64, 0.39 in groups of
+// 8 and 0.80 with one lane per ray walking depth first (rays in Morton order);
+// the 65,536-face torus knot (4,124 rays) 0.048 ms against 0.075 in groups of
+// 64 and 0.071 in groups of 8; the 256 merged icospheres (1.2 M rays) 2.32 ms
+// against 2.40 depth first. A Morton order of the rays gained no more than
+// its sort cost.
+constexpr int RAY_FEW = 2048;
+constexpr int RAY_GROUP = 16;
 __device__ __forceinline__ int ray_group(int rays, int target) {
-  if (rays <= RAY_FEW && (long)64 * rays <= (long)target) return 64;
-  if ((long)RAY_GROUP * rays <= (long)target) return RAY_GROUP;
-  return 1;
+  return rays <= RAY_FEW && (long)64 * rays <= (long)target ? 64 : RAY_GROUP;
 }
-// Lanes of a depth-first ray walk (one leaf subtree each): one per ray when the
-// rays take a lane each; else a whole wave, for the rays k_ray_bfs hands over.
-__device__ __forceinline__ int walk_lanes(const Tree &tree, int group) {
-  if (group == 1) return 1;
-  return tree.leaves < 64 ? tree.leaves : 64;
-}
+// Lanes of the depth-first walk of a ray that k_ray_bfs hands over: a whole
+// wave, one leaf subtree each.
+__device__ __forceinline__ int walk_lanes(const Tree &tree) { return tree.leaves < 64 ? tree.leaves : 64; }
 
 // Sum over the aligned group of K lanes (every lane of the wave active).
 __device__ __forceinline__ long long group_sum(long long v, int k) {
@@ -570,18 +566,13 @@ __device__ __forceinline__ long long group_sum(long long v, int k) {
   return v;
 }
 
-// Ray leaders, compacted, with the Morton codes of their samples. When rays
-// take one lane each, the leaders are then sorted by code, so the lanes of a
-// wave walk the same hierarchy nodes (the reference sorts its queries by
-// coordinates, events.py:645-678); when a wave walks each ray (lanes over
-// leaf subtrees) the order does not matter and the sort is skipped: the last
-// block to finish sets the sort's count to the leaders or to zero. A block
-// claims slots for the leaders among 1,024 patches with one atomic (returning
-// atomics on one word serialise at about 15 ns each: one per wave cost the
-// shells 0.19 ms).
+// Ray leaders (the patches whose group they lead), compacted in patch order.
+// A block claims slots for the leaders among 1,024 patches with one atomic
+// (returning atomics on one word serialise at about 15 ns each: one per wave
+// cost the shells 0.19 ms).
 constexpr int KEY_ITEMS = 4;
-__global__ __launch_bounds__(256) void k_ray_keys(Ws w) {
-  __shared__ int last, block_base, wave_total[4];
+__global__ __launch_bounds__(256) void k_ray_leaders(Ws w) {
+  __shared__ int block_base, wave_total[4];
   const bool ok = w.status[ST_ERR] == 0;
   const int P = npolys(w), wave = threadIdx.x >> 6;
   for (long base = (long)blockIdx.x * 256 * KEY_ITEMS; base < P; base += (long)gridDim.x * 256 * KEY_ITEMS) {
@@ -608,24 +599,9 @@ __global__ __launch_bounds__(256) void k_ray_keys(Ws w) {
 #pragma unroll
     for (int k = 0; k < KEY_ITEMS; ++k) {
       if (!lead[k]) continue;
-      long i = base + threadIdx.x + 256L * k;
-      double2 x = w.samples[i];
-      w.ray_keys[slot] = morton2((float)x.x, (float)x.y);
-      w.ray_ids[slot] = (int)i;
-      ++slot;
+      w.ray_ids[slot++] = (int)(base + threadIdx.x + 256L * k);
     }
     __syncthreads();
-  }
-  // Each block's leader count is an atomic that has returned before its
-  // arrival is counted, so the last block reads the total from the counter
-  // itself: no fence is needed (the keys reach the sort at the kernel's end).
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(w.control + C_RAYKEY_DONE, 1) == (int)gridDim.x - 1;
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    int leaders = load_i(w.control + C_LEADERS);
-    w.control[C_RAYSORT] = ray_group(leaders, w.ray_lanes_target) == 1 ? leaders : 0;
-    w.control[C_RAYKEY_DONE] = 0;
   }
 }
 
@@ -712,18 +688,15 @@ __device__ void add_ray_counts(const Ws &w, const RayCounts &rc) {
   }
 }
 
-// ray_hits (event_kernels.py:196-245), leaders only: the first C_LEADERS
-// entries of the Morton-sorted ray order, one lane each. When the rays take
-// groups of lanes, k_ray_bfs, launched before, has taken them, and this launch
-// walks only those it listed (their level lists outgrew its LDS), a wave each.
+// ray_hits (event_kernels.py:196-245) for the rays k_ray_bfs, launched
+// before, listed because their level lists outgrew its LDS: a wave each, one
+// leaf subtree per lane.
 __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
   if (w.status[ST_ERR] != 0) return;
-  const int leaders = load_i(w.control + C_LEADERS);
   const Tree tree = w.source_tree.tree();
-  const int G = ray_group(leaders, w.ray_lanes_target), K = walk_lanes(tree, G);
-  const bool listed = G > 1;
-  const int n = listed ? load_i(w.control + C_RAYFALL) : leaders;
-  const int *order = listed ? w.ray_ids_alt : w.ray_ids;
+  const int K = walk_lanes(tree);
+  const int n = load_i(w.control + C_RAYFALL);
+  const int *order = w.ray_ids_alt;
   HitStage stage = hit_stage();
   const int per_wave = 64 / K;
   const long waves = (gridDim.x * blockDim.x) >> 6;
@@ -734,7 +707,8 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
   add_ray_counts(w, rc);
 }
 
-// The same rays breadth first, a group of G lanes (64 or 16) each: each level
+// ray_hits (event_kernels.py:196-245), leaders only: the first C_LEADERS
+// entries of ray_ids, breadth first, a group of G lanes (16, or 64) each: each level
 // of the source hierarchy's four-wide walk is expanded by the group at once (a
 // lane per grandchild box, the overlapping ones appended to the next level's
 // list in LDS), then the group's lanes test the leaves of the overlapping
@@ -743,8 +717,7 @@ __global__ __launch_bounds__(256) void k_ray_hits(Ws w) {
 // walk. The candidates are the same leaves (those whose boxes meet the ray
 // box), so the hits and counts are too; the hit order is restored by the sort.
 // A ray whose list outgrows its share of RAY_FRONTIER is listed (its patch
-// index, in ray_ids_alt, which the skipped Morton sort leaves free) for
-// k_ray_hits. Loops run while any group of the wave has work (the ballots and
+// index, in ray_ids_alt) for k_ray_hits. Loops run while any group of the wave has work (the ballots and
 // the hit stage need every lane).
 constexpr int RAY_FRONTIER = 512;
 constexpr int RAY_BFS_BLOCKS = 2048;  // 8,192 waves
@@ -755,7 +728,6 @@ __global__ __launch_bounds__(256) void k_ray_bfs(Ws w) {
   const int n = load_i(w.control + C_LEADERS);
   const Tree tree = w.source_tree.tree();
   const int G = ray_group(n, w.ray_lanes_target);
-  if (G == 1) return;
   HitStage stage = hit_stage();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int per_wave = 64 / G, g = lane / G, gl = lane % G, cap = RAY_FRONTIER / per_wave;
@@ -849,7 +821,7 @@ __global__ __launch_bounds__(256) void k_heavy_hits(Ws w, int workers) {
   int jobs = ok ? load_i(w.control + C_HEAVY) : 0;
   if (jobs > w.cap.regions) jobs = w.cap.regions;
   // leaves per lane subtree, with the lanes per ray that k_ray_hits chose
-  const int W = tree.leaves / walk_lanes(tree, ray_group(load_i(w.control + C_LEADERS), w.ray_lanes_target));
+  const int W = tree.leaves / walk_lanes(tree);
   // a job split into more parts re-enters that many subtrees from their
   // roots: split only while the split jobs still fit the heavy lane budget
   int parts = W;
@@ -1387,10 +1359,8 @@ void enqueue_rank(const Ws &w, hipStream_t s, Timeline *t) {
   }
   {
     long kg = ((long)w.cap.regions + 256 * KEY_ITEMS - 1) / (256 * KEY_ITEMS);
-    hipLaunchKernelGGL(k_ray_keys, dim3((unsigned)std::max(1L, std::min(kg, 1024L))), dim3(256), 0, s, w);
+    hipLaunchKernelGGL(k_ray_leaders, dim3((unsigned)std::max(1L, std::min(kg, 1024L))), dim3(256), 0, s, w);
   }
-  radix_sort(s, w.ray_keys, w.ray_ids, w.ray_keys_alt, w.ray_ids_alt, w.control + C_RAYSORT, 0,
-             w.cap.regions, 0, 32, w.sort);
   mark(t, s, "samples_groups");
   hipLaunchKernelGGL(k_ray_bfs, dim3(std::min(stride_grid(w.cap.regions).x, (unsigned)RAY_BFS_BLOCKS)), dim3(256), 0,
                      s, w);

@@ -103,8 +103,7 @@ struct Ws {
   int *group_counts;       // [regions]
   int *group_offsets;      // [regions]
   int *group_fill;         // [regions] segment fill cursors
-  uint64_t *ray_keys, *ray_keys_alt;  // [regions] Morton keys of ray samples
-  int *ray_ids, *ray_ids_alt;         // [regions] rays in Morton order
+  int *ray_ids, *ray_ids_alt;         // [regions] ray leaders; rays left to the depth-first walk
   int *medium;             // [regions] groups sorted by one workgroup
   int4 *heavy;             // [regions] (ray, first unvisited leaf, lane subtree root)
   // rectangle certification of shared rays over 32 x 32 window cells

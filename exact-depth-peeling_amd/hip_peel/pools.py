@@ -59,7 +59,7 @@ TARGETS = (
 )
 # approximate device bytes per element of each pool (for reports only; the
 # workspace reports its exact size as ``device_bytes``)
-ELEMENT_BYTES = {"frontier": 532, "regions": 666, "hits": 40, "vertices": 126, "triangles": 128}
+ELEMENT_BYTES = {"frontier": 532, "regions": 650, "hits": 40, "vertices": 126, "triangles": 128}
 MIN_CAPACITY = 64
 # Closed meshes of at least this many faces get a per-face floor on every
 # predicted pool (PoolModel.floor).

@@ -11,9 +11,8 @@ switches, face-count octave) in a JSON file, so a workload is tuned once.
 Fields (``include/peel355.h`` ``PeelTuning``):
 
 * ``persistent_blocks_per_cu``: blocks of the subdivision work queue per CU;
-* ``ray_lanes_target``: lanes in flight allowed for breadth-first rays: 64
-  lanes per ray while 64 x rays fits, else 16 while 16 x rays fits, else
-  one lane per ray (depth first);
+* ``ray_lanes_target``: lanes in flight allowed for rays of a whole wave each
+  (at most 2,048 rays, 64 x rays within it); other rays take 16 lanes each;
 * ``heavy_lanes``: lane budget of the subtree jobs of heavy rays and beams;
 * ``long_edge_cells``: grid cells beyond which a polygon edge is noded by a
   whole wave instead of one lane;
@@ -29,7 +28,7 @@ from pathlib import Path
 
 CANDIDATES = {
     "persistent_blocks_per_cu": (1, 2, 3, 4),
-    "ray_lanes_target": (65536, 131072, 262144, 524288, 1048576, 4194304, 16777216),
+    "ray_lanes_target": (1024, 131072, 4194304),
     "heavy_lanes": (16384, 32768, 65536, 131072, 262144),
     "long_edge_cells": (16, 32, 48, 96, 192),
     "leaf_bucket_log2": (1, 2, 3, 4),

@@ -130,7 +130,7 @@ int peel_profile(PeelWorkspace *ws, int32_t *count, float *ms, char *names, int3
  * Outputs do not depend on these values, only the time. */
 typedef struct PeelTuning {
   int32_t persistent_blocks_per_cu; /* k_subdivide work-queue blocks per CU, 1..max_blocks_per_cu */
-  int32_t ray_lanes_target;         /* lanes in flight allowed for rays in groups of 64 lanes, else 16, else 1 lane per ray; >= 64 */
+  int32_t ray_lanes_target;         /* lanes in flight allowed for rays of 64 lanes each (at most 2,048 rays), else 16 lanes per ray; >= 64 */
   int32_t heavy_lanes;              /* lane budget of the heavy-ray and beam subtree jobs, >= 1024 */
   int32_t long_edge_cells;          /* grid cells beyond which a polygon edge is noded by a wave, >= 1 */
   int32_t leaf_bucket_log2;         /* ray walks test 2^n (or 2^(n+1)) hierarchy leaves together, 0..4 */

@@ -51,21 +51,20 @@ def test_outputs_do_not_depend_on_tuning(filter_fp32):
 
 @pytest.mark.parametrize("filter_fp32", [False, True])
 def test_ray_walks_agree(filter_fp32):
-    """Rays in groups of 64 lanes and of 16 (breadth first, k_ray_bfs) and one
-    lane per ray (depth first, k_ray_hits) give the same patches, triangles
-    and candidate-test totals."""
+    """Rays in groups of 64 lanes and of 16 (breadth first, k_ray_bfs) give the
+    same patches and triangles."""
     print(f"[tune] ray walk modes fp32={filter_fp32}", flush=True)
     p = knot_peeler(filter_fp32=filter_fp32)
     p.run()
     rays = p.read_status()["ray_groups"]
-    assert rays > 8
+    assert 8 < rays <= 2048
     results = []
-    for target in (1 << 30, 24 * rays, 64):  # groups of 64, of 16, one lane each
+    for target in (1 << 30, 64):  # groups of 64, of 16
         p.set_tuning(ray_lanes_target=target)
         p.run()
         info, r, s, l, uv, w, ts, tl = gpu_outputs(p)
         results.append((canonical_patches(r, s, l), canonical_triangles(uv, w, ts, tl)))
-    assert results[0] == results[1] == results[2]
+    assert results[0] == results[1]
 
 
 def test_autotune_and_cache(tmp_path):
