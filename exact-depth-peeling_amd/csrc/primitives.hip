@@ -12,7 +12,8 @@
 //             walking back to the nearest published prefix, publishes its own
 //             prefix and scatters stably;
 //   rs_done   copies back after an odd number of moving passes and clears the
-//             counters, tickets and look-back words for the next sort.
+//             counters, tickets and look-back words for the next sort (the
+//             hierarchy builds do this in h_leaves instead: rs_finish).
 // Tickets are taken in execution order, so a tile waits only on tiles whose
 // blocks are already running and publish their counts before they wait
 // themselves; every wait is bounded (ERR_WAIT). A pass is trivial when one
@@ -234,20 +235,20 @@ __global__ __launch_bounds__(RS_THREADS) void rs_sweep(const uint64_t *k0, const
   }
 }
 
-__global__ void rs_done(uint64_t *k0, int *v0, const uint64_t *k1, const int *v1, const int *count,
-                        int count_static, int capacity, int passes, const int *flags, int *counts, int *tickets,
-                        unsigned *tiles, int max_blocks) {
-  int n = live_count(count, count_static, capacity);
+// The end of a sort, for the grid thread `first` of `stride`: the ids (and,
+// with `copy_keys`, the keys) of the items it owns move back after an odd
+// number of moving passes, and the look-back words, digit counts and tickets
+// are left zero for the next sort.
+__device__ void rs_finish(uint64_t *k0, int *v0, const uint64_t *k1, const int *v1, int n, int passes,
+                          const int *flags, int *counts, int *tickets, unsigned *tiles, int max_blocks,
+                          bool copy_keys, long first, long stride) {
   int sel = 0;
   for (int p = 0; p < passes; ++p) sel ^= flags[p];
-  const long stride = (long)gridDim.x * blockDim.x;
-  const long first = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (sel)
     for (long i = first; i < n; i += stride) {
-      k0[i] = k1[i];
+      if (copy_keys) k0[i] = k1[i];
       v0[i] = v1[i];
     }
-  // leave the scratch zero for the next sort
   const long used = (long)((n + RS_TILE - 1) / RS_TILE) * 256;
   for (int p = 0; p < passes; ++p)
     for (long i = first; i < used; i += stride) tiles[(long)p * max_blocks * 256 + i] = 0u;
@@ -255,9 +256,16 @@ __global__ void rs_done(uint64_t *k0, int *v0, const uint64_t *k1, const int *v1
   if (first < passes) tickets[first] = 0;
 }
 
+__global__ void rs_done(uint64_t *k0, int *v0, const uint64_t *k1, const int *v1, const int *count,
+                        int count_static, int capacity, int passes, const int *flags, int *counts, int *tickets,
+                        unsigned *tiles, int max_blocks) {
+  rs_finish(k0, v0, k1, v1, live_count(count, count_static, capacity), passes, flags, counts, tickets, tiles,
+            max_blocks, true, (long)blockIdx.x * blockDim.x + threadIdx.x, (long)gridDim.x * blockDim.x);
+}
+
 void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int *alt_ids,
                 const int *count, int count_static, int capacity, int bit_begin, int bit_end,
-                const SortScratch &scratch) {
+                const SortScratch &scratch, bool finish) {
   int blocks = (capacity + RS_TILE - 1) / RS_TILE;
   if (blocks < 1 || bit_end <= bit_begin) return;
   int first = bit_begin / 8, last = (bit_end + 7) / 8;
@@ -270,6 +278,7 @@ void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int
     hipLaunchKernelGGL(rs_sweep, dim3(blocks), dim3(RS_THREADS), 0, s, keys, ids, alt_keys, alt_ids, count,
                        count_static, capacity, (first + pass) * 8, pass, scratch.counts, scratch.tickets,
                        scratch.tiles, scratch.max_blocks, scratch.flags, scratch.status);
+  if (!finish) return;  // the caller's next kernel runs rs_finish
   int grid = blocks * 4 < 4096 ? blocks * 4 : 4096;
   hipLaunchKernelGGL(rs_done, dim3(grid), dim3(256), 0, s, keys, ids, alt_keys, alt_ids, count, count_static,
                      capacity, passes, scratch.flags, scratch.counts, scratch.tickets, scratch.tiles,
@@ -385,7 +394,10 @@ void scan_exclusive(hipStream_t s, const int *in, int *out, const int *count, in
 // ---------------------------------------------------------------------------
 // Morton hierarchy (warp_peel/kernels.py:67-116, solver.py:34-89).
 // Morton keys of the live boxes; thread 0 also sizes the tree: the power of two
-// at least the live count.
+// at least the live count. (Counting the sort's digits here, on at most 256
+// blocks as rs_count does, saved its launch on the 65,536-face torus knot but
+// cost the large hierarchies more: the keys' producer then walks its boxes in
+// long per-block loops instead of one box per thread.)
 __global__ void h_keys(const Box *boxes, const int *count, int count_static, int capacity,
                        int layered, uint64_t *keys, int *ids, int *leaves) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -414,12 +426,20 @@ __global__ void h_keys(const Box *boxes, const int *count, int count_static, int
   keys[i] = key;
 }
 
+// The leaves in Morton order. Each thread first finishes the sort for the
+// item it owns (the grid covers the tree's leaves, at least the live count)
+// and then reads that item's id.
 __global__ void h_leaves(const Box *boxes, const int *count, int count_static, int capacity,
-                         const int *ids, const int *leaves, Box *nodes, LeafGather g) {
+                         int *ids, const int *alt_ids, int passes, SortScratch sort, const int *leaves,
+                         Box *nodes, LeafGather g) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = live_count(count, count_static, capacity);
+  // the items, look-back words and counts end below max(n, passes * 256)
+  if (i < n || i < passes * 256)
+    rs_finish(nullptr, ids, nullptr, alt_ids, n, passes, sort.flags, sort.counts, sort.tickets, sort.tiles,
+              sort.max_blocks, false, i, (long)gridDim.x * blockDim.x);
   int size = *leaves;
   if (i >= size) return;
-  int n = live_count(count, count_static, capacity);
   Box b;
   b.lo = make_float4(1e30f, 1e30f, 1e30f, 0.f);
   b.hi = make_float4(-1e30f, -1e30f, -1e30f, 0.f);
@@ -478,13 +498,13 @@ void build_hierarchy(hipStream_t s, const Box *boxes, const int *count, int coun
                      int capacity, bool layered, Hierarchy &h, const SortScratch &scratch,
                      const LeafGather &gather) {
   int cap = capacity < h.size ? capacity : h.size;
+  const int bits = layered ? 64 : 32;
   int grid = (cap + 255) / 256;
   hipLaunchKernelGGL(h_keys, dim3(grid < 1 ? 1 : grid), dim3(256), 0, s, boxes, count, count_static,
                      cap, (int)layered, h.keys, h.ids, h.leaves);
-  radix_sort(s, h.keys, h.ids, h.alt_keys, h.alt_ids, count, count_static, cap, 0, layered ? 64 : 32,
-             scratch);
-  hipLaunchKernelGGL(h_leaves, dim3((h.size + 255) / 256), dim3(256), 0, s, boxes, count,
-                     count_static, cap, h.ids, h.leaves, h.nodes, gather);
+  radix_sort(s, h.keys, h.ids, h.alt_keys, h.alt_ids, count, count_static, cap, 0, bits, scratch, false);
+  hipLaunchKernelGGL(h_leaves, dim3((h.size + 255) / 256), dim3(256), 0, s, boxes, count, count_static, cap,
+                     h.ids, h.alt_ids, (bits + 7) / 8, scratch, h.leaves, h.nodes, gather);
   for (int stage = 0; (1L << (HIER_STAGE_LEVELS * stage)) < h.size; ++stage) {
     long blocks = (h.size >> (HIER_STAGE_LEVELS * stage)) / HIER_GROUP;
     hipLaunchKernelGGL(h_levels, dim3((unsigned)(blocks < 1 ? 1 : blocks)), dim3(256), 0, s, h.nodes, h.leaves,

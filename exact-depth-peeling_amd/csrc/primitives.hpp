@@ -25,9 +25,11 @@ struct SortScratch {
 // Sort (keys, ids) in place by the key bits [bit_begin, bit_end), stably.
 // count: device pointer (clamped to capacity) or nullptr to use count_static.
 // alt_keys / alt_ids are ping-pong buffers of the same capacity.
+// With finish == false the caller's next kernel must run rs_finish over the
+// items (the ids after an odd number of moving passes, and the scratch).
 void radix_sort(hipStream_t s, uint64_t *keys, int *ids, uint64_t *alt_keys, int *alt_ids,
                 const int *count, int count_static, int capacity, int bit_begin, int bit_end,
-                const SortScratch &scratch);
+                const SortScratch &scratch, bool finish = true);
 
 // Zero between scans: allocation clears it and every scan's last tile clears
 // what the scan used.
