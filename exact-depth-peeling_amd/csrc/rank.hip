@@ -528,18 +528,14 @@ __device__ __forceinline__ void ray_box(double2 x, float4 &lo, float4 &hi) {
   hi = make_float4((float)x.x + 2.0e-7f, (float)x.y + 2.0e-7f, 1.0e30f, 0.f);
 }
 
-// K lanes per ray, chosen on the device from the number of rays: 64 when at
-// most 8,192 rays would leave most of the chip idle (65,536-face torus knot:
-// 4,116 rays, ray stage 0.90 -> 0.42 ms), 1 when there are more (717,740-face
-// shells: one lane per ray keeps 64 Morton-adjacent rays per wave walking the
-// same nodes; 4 lanes per ray measured 2.5x slower there). The K lanes of a ray split the source
-// hierarchy into its K subtrees at depth log2 K and each walks its own, so a
-// ray's candidates are visited in parallel and lanes whose subtree box misses
-// the ray stop after one node. A lane that reaches HANDOFF candidates hands the
-// unvisited rest of its subtree to the subtree-parallel kernel (or keeps
-// walking if that queue is full). The reference counts a group as heavy once
-// it passes 32 candidates (event_kernels.py:222-245): that counter keeps its
-// meaning.
+// The depth-first walk of a ray, for the rays whose breadth-first level lists
+// outgrow LDS: K lanes (a whole wave) split the source hierarchy into its K
+// subtrees at depth log2 K and each walks its own, so a ray's candidates are
+// visited in parallel and lanes whose subtree box misses the ray stop after
+// one node. A lane that reaches HANDOFF candidates hands the unvisited rest of
+// its subtree to the subtree-parallel kernel (or keeps walking if that queue
+// is full). The reference counts a group as heavy once it passes 32
+// candidates (event_kernels.py:222-245): that counter keeps its meaning.
 constexpr int HANDOFF = 32;
 
 // Lanes per ray of the breadth-first walk (k_ray_bfs): 16, or a whole wave
