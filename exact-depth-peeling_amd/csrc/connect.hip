@@ -62,27 +62,48 @@ __device__ void edge_owners_one(const Ws &w, int i) {
   int base = w.voffsets[i];
   if (base + n > w.cap.vertices) return;
   int source = w.out_src[i];
-  int qn = w.src_n[source];
   long R = w.cap.regions;
+  // The source's internal edges, in the order they are tried, with their
+  // endpoints and owner keys, read once for all the patch's edges (the edge
+  // stores below would otherwise make every edge read them again). A clipped
+  // source has none; an unclipped one is a triangle (at most three).
+  double2 eu[3], ev[3];
+  unsigned long long ekey[3];
+  int cand = 0;
+  if (w.clipped[source] == 0) {
+    const int qn = w.src_n[source];
+    const bool back = w.facing[source] < 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      if (j >= qn) continue;
+      const int edge = back ? (4 - j) % 3 : j;
+      if (w.masks[3 * source + edge] != 0) continue;
+      const double2 u = w.src_xy[(long)j * w.nf + source], v = w.src_xy[(long)((j + 1) % qn) * w.nf + source];
+      const int ca = w.canonical[w.faces[3 * source + edge]];
+      const int cb = w.canonical[w.faces[3 * source + (edge + 1) % 3]];
+      const unsigned long long key = ((unsigned long long)(uint32_t)min(ca, cb) << 32) | (uint32_t)max(ca, cb);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (c == cand) {
+          eu[c] = u;
+          ev[c] = v;
+          ekey[c] = key;
+        }
+      ++cand;
+    }
+  }
   for (int k = 0; k < n; ++k) {
     double2 a = w.out_xy[k * R + i], b = w.out_xy[((k + 1) % n) * R + i];
     unsigned long long owner = 0x8000000000000000ull | (unsigned long long)(uint32_t)source;
-    if (w.clipped[source] == 0) {
-      for (int j = 0; j < qn; ++j) {
-        int edge = w.facing[source] < 0 ? (4 - j) % 3 : j;
-        if (w.masks[3 * source + edge] == 0) {
-          double2 u = w.src_xy[(long)j * w.nf + source], v = w.src_xy[(long)((j + 1) % qn) * w.nf + source];
-          // identity checks also avoid cancellation at shared endpoints
-          if ((same(a, u) || same(a, v) || cross3(u, v, a) == 0.0) &&
-              (same(b, u) || same(b, v) || cross3(u, v, b) == 0.0)) {
-            int e = 3 * source + edge;
-            int f = e / 3, jj = e % 3;
-            int ca = w.canonical[w.faces[3 * f + jj]];
-            int cb = w.canonical[w.faces[3 * f + (jj + 1) % 3]];
-            owner = ((unsigned long long)(uint32_t)min(ca, cb) << 32) | (uint32_t)max(ca, cb);
-            break;
-          }
-        }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (c >= cand) break;
+      const double2 u = eu[c], v = ev[c];
+      // identity checks also avoid cancellation at shared endpoints
+      if ((same(a, u) || same(a, v) || cross3(u, v, a) == 0.0) &&
+          (same(b, u) || same(b, v) || cross3(u, v, b) == 0.0)) {
+        owner = ekey[c];
+        break;
       }
     }
     int slot = base + k;
