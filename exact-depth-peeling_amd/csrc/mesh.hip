@@ -52,13 +52,23 @@ __device__ void scatter_one(const Ws &w, int i) {
   int layer = w.out_layer[i];
   long R = w.cap.regions;
   double2 center = make_double2(0.0, 0.0);
-  for (int j = 0; j < n; ++j) {
-    double2 v = w.out_xy[j * R + i];
-    center.x += v.x / (double)n;
-    center.y += v.y / (double)n;
-    w.bverts[base + j] = v;
-    w.blayers[base + j] = layer;
-    w.tasks[base + j] = make_int2(i, j | n << 16);
+  // eight vertices are loaded before any is stored (a store between two loads
+  // would make each load wait for the one before)
+  for (int j0 = 0; j0 < n; j0 += 8) {
+    double2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (j0 + r < n) v[r] = w.out_xy[(j0 + r) * R + i];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int j = j0 + r;
+      if (j >= n) continue;
+      center.x += v[r].x / (double)n;
+      center.y += v[r].y / (double)n;
+      w.bverts[base + j] = v[r];
+      w.blayers[base + j] = layer;
+      w.tasks[base + j] = make_int2(i, j | n << 16);
+    }
   }
   w.centers[i] = center;
 }

@@ -374,9 +374,17 @@ __global__ void k_leaf_sources(Ws w) {
   if (k >= w.nf) return;
   int id = h.ids[k];
   int n = 0;
+  // the ring is loaded whole before it is stored (a store between two loads
+  // would make each load wait for the one before)
+  double2 ring[SRCV];
   if (id < w.nf && w.status[ST_ERR] == 0) {
     n = w.src_n[id];
-    for (int j = 0; j < n; ++j) w.leaf_xy[8L * k + j] = w.src_xy[(long)j * w.nf + id];
+#pragma unroll
+    for (int j = 0; j < SRCV; ++j)
+      if (j < n) ring[j] = w.src_xy[(long)j * w.nf + id];
+#pragma unroll
+    for (int j = 0; j < SRCV; ++j)
+      if (j < n) w.leaf_xy[8L * k + j] = ring[j];
     w.leaf_score[k] = w.scores[id];
   }
   w.leaf_n[k] = n;
@@ -388,8 +396,10 @@ __global__ void k_leaf_sources(Ws w) {
     // binary32 copy of the ring and the static forward error bound of its
     // orientation tests against any point a ray query can bring to this leaf
     double lx = 1e300, ly = 1e300, hx = -1e300, hy = -1e300;
-    for (int j = 0; j < n; ++j) {
-      double2 v = w.leaf_xy[8L * k + j];
+#pragma unroll
+    for (int j = 0; j < SRCV; ++j) {
+      if (j >= n) continue;
+      const double2 v = ring[j];
       w.leaf_xy32[8L * k + j] = make_float2((float)v.x, (float)v.y);
       lx = fmin(lx, v.x), ly = fmin(ly, v.y), hx = fmax(hx, v.x), hy = fmax(hy, v.y);
     }
