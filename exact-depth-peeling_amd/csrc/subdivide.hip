@@ -58,7 +58,9 @@ __device__ __forceinline__ double2 vtx(const Slot &s, int j) { return s.xy[(long
 // vertex-major so the 64 lanes of a wave read vertex j of their rings from
 // consecutive banks; the rare longer ring keeps the rest in its slot (slots
 // are never reused within a run). 288 B of LDS per lane (six vertices a
-// ring: measured faster than twelve on both the torus knot and the shells).
+// ring: measured faster than twelve on both the torus knot and the shells,
+// and faster than four or three, which fit three or four blocks a CU instead
+// of two but send more ring vertices to their slots).
 constexpr int SUB_THREADS = 256;
 constexpr int SUB_LDS = 6;
 constexpr int SUB_RING = SUB_LDS * SUB_THREADS;  // LDS words of one ring for the block
